@@ -1,0 +1,202 @@
+/*
+ * dpcull_hip.h -- C ABI of the MI355X (gfx950) backend for nvpro-pipeline's
+ * scene-update path: world-matrix propagation in the transform tree and
+ * bounding-box frustum culling.
+ *
+ * The reference exposes this path as C++ virtual interfaces:
+ *   dp::culling::Manager          (dp/culling/Manager.h:108-143)
+ *   dp::transform::Tree           (dp/transform/Tree.h:42-129)
+ * Those interfaces are mirrored in C++ by include/dpcull/Manager.hpp and
+ * include/dpcull/Tree.hpp, which call only the functions below.  Each entry
+ * point names the reference code it replaces.
+ *
+ * Conventions
+ *   - Every function returns DPC_OK (0) or a DPC_ERR_* code; nothing throws
+ *     across this boundary.  dpc_last_error() returns the calling thread's
+ *     last message.
+ *   - "d_" pointers are device memory (dpc_malloc or any HIP allocation on the
+ *     current device); other pointers are host memory borrowed for the call.
+ *   - Matrices are 4x4 FP32, row-major, row-vector convention (v' = v * M), as
+ *     dp::math::Mat44f (dp/math/Matmnt.h:199).  On the device they live in
+ *     "row planes": plane r (r = 0..3) holds row r of every matrix as float4,
+ *     plane r starts at d_planes + r * ld * 4 floats.  ld >= matrix count.
+ *   - Object boxes live in two device arrays: box_a[i] = (lower.x, lower.y,
+ *     lower.z, extent.x) as float4 and box_b[i] = (extent.y, extent.z) as
+ *     float2, where extent = upper - lower (ManagerBitSet.cpp:99-109).
+ *   - A visibility mask of n objects is ceil(n/64) 64-bit words, little-endian,
+ *     bit i = object i (dp::util::BitArray layout read as uint32 words,
+ *     dp/culling/cpu/src/ManagerImpl.cpp:477,517).  Bits >= n are 0.  Mask
+ *     pointers must be 8-byte aligned.
+ *   - stream arguments are hipStream_t values passed as void* (NULL = the
+ *     null stream).  Launch functions are asynchronous on that stream.
+ *   - FP32 arithmetic follows the reference's evaluation order with no fused
+ *     multiply-add, so masks are bit-identical to dp::culling::cpu and world
+ *     matrices to dp::transform::Tree::compute.
+ */
+#ifndef DPCULL_HIP_H
+#define DPCULL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPC_OK 0
+#define DPC_ERR_INVALID_ARGUMENT 1
+#define DPC_ERR_HIP 2
+#define DPC_ERR_OUT_OF_MEMORY 3
+#define DPC_ERR_OVER_BUDGET 4
+#define DPC_ERR_NOT_FOUND 5
+#define DPC_ERR_STATE 6
+
+#define DPC_ABI_VERSION 1
+
+/* ---- runtime plumbing (no reference counterpart: the reference's CPU path
+ *      needs none, its GL sibling uses dp/gl buffers, opengl/src/GroupImpl.cpp:68-183) */
+int dpc_abi_version(void);
+const char* dpc_last_error(void);
+int dpc_device_info(int device, char* name, size_t name_len, int* cu_count, int* clock_khz,
+                    size_t* total_mem, int* l2_bytes);
+int dpc_set_device(int device);
+int dpc_malloc(void** d_ptr, size_t bytes);
+int dpc_free(void* d_ptr);
+int dpc_host_alloc(void** h_ptr, size_t bytes); /* pinned host memory */
+int dpc_host_free(void* h_ptr);
+int dpc_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream);
+int dpc_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream);
+int dpc_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream);
+/* Fills n 32-bit words with value using a kernel (safe inside graph capture). */
+int dpc_fill_u32(uint32_t* d_dst, uint32_t value, size_t n, void* stream);
+int dpc_stream_create(void** stream);
+int dpc_stream_destroy(void* stream);
+int dpc_stream_sync(void* stream);
+int dpc_device_sync(void);
+int dpc_event_create(void** event);
+int dpc_event_destroy(void* event);
+int dpc_event_record(void* event, void* stream);
+int dpc_event_sync(void* event);
+int dpc_event_elapsed_ms(void* start, void* stop, float* ms);
+/* hipGraph capture of work issued on a stream, and replay. */
+int dpc_graph_begin(void* stream);
+int dpc_graph_end(void* stream, void** graph_exec);
+int dpc_graph_launch(void* graph_exec, void* stream);
+int dpc_graph_destroy(void* graph_exec);
+
+/* ---- layout conversion ------------------------------------------------------
+ * Reference matrices are an array with a byte stride
+ * (GroupBitSet::setMatrices, dp/culling/src/GroupBitSet.cpp:121-135;
+ * dp::transform::Tree::getWorldMatrices, dp/transform/Tree.h:81).
+ * d_aos holds count matrices of 64 bytes at stride_bytes apart (a copy of
+ * that array on the device); they are written to row planes d_planes[0..count). */
+int dpc_matrices_from_aos(const void* d_aos, size_t stride_bytes, size_t count, float* d_planes, size_t ld,
+                          void* stream);
+/* Inverse: row planes -> AoS, optionally through an index map
+ * (d_aos[k] = plane matrix d_slot[k]; d_slot NULL = identity). */
+int dpc_matrices_to_aos(const float* d_planes, size_t ld, const uint32_t* d_slot, size_t count, void* d_aos,
+                        size_t stride_bytes, void* stream);
+/* Scatter of dirty matrices ("delta"): plane matrix d_idx[k] = d_src[k] (16 floats
+ * each).  Device-side form of Tree::updateLocalMatrix (dp/transform/Tree.h:85)
+ * and of rewriting a matrix then calling Manager::groupMatrixChanged
+ * (dp/culling/Manager.h:125). */
+int dpc_matrices_scatter(float* d_planes, size_t ld, const uint32_t* d_idx, const float* d_src, size_t count,
+                         void* stream);
+/* ManagerBitSet::objectSetBoundingBox (dp/culling/src/ManagerBitSet.cpp:99-109)
+ * for n boxes at once: d_lower_upper is n x 6 floats (lower.xyz, upper.xyz);
+ * extent = upper - lower in FP32. */
+int dpc_boxes_from_lower_upper(const float* d_lower_upper, size_t n, float* d_box_a, float* d_box_b,
+                               void* stream);
+
+/* ---- frustum culling --------------------------------------------------------
+ * dp::culling::cpu::ManagerImpl::cull (dp/culling/cpu/src/ManagerImpl.cpp:467-518)
+ * with GroupCPU::updateOBBs (:114-162) fused in: for each object i < n,
+ *   M     = matrix d_xform[i] (or matrix i when d_xform is NULL)
+ *   bit i = isVisible(vp, OBB(box_i, M))            (:199-229, :263-289)
+ * vp is host memory (16 floats), passed by value to the kernel. */
+int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+             const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, void* stream);
+
+/* ---- batched multi-frustum culling -------------------------------------------
+ * Many cull(group, result, viewProjection) calls (dp/culling/Manager.h:139) in
+ * one persistent launch.  A group is a contiguous object range; a pair asks
+ * for (frustum, group) and names where its mask starts in d_masks (in 64-bit
+ * words).  Each object's matrix and box are read once per launch and tested
+ * against every frustum paired with its group. */
+typedef struct dpc_group_range {
+  uint64_t first_object;
+  uint64_t count;
+} dpc_group_range;
+
+typedef struct dpc_cull_pair {
+  uint32_t frustum;
+  uint32_t group;
+  uint64_t mask_word_offset;
+} dpc_cull_pair;
+
+typedef struct dpc_batch_config {
+  uint32_t objects_per_lane; /* 1, 2 or 4: a wave owns 64*objects_per_lane objects */
+  uint32_t block_threads;    /* 64, 128, 256 */
+  uint32_t waves_per_cu;     /* persistent grid size = CUs * waves_per_cu / waves per block */
+  uint32_t max_frusta_per_item; /* split a tile's frustum list into chunks of this size (0 = no split) */
+} dpc_batch_config;
+
+typedef struct dpc_batch_plan dpc_batch_plan;
+
+/* Builds the work list (tiles sorted by cost) and uploads it.  cfg NULL = defaults. */
+int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const dpc_cull_pair* pairs,
+                          size_t n_pairs, uint32_t n_frusta, const dpc_batch_config* cfg, dpc_batch_plan** plan);
+int dpc_batch_plan_destroy(dpc_batch_plan* plan);
+/* Total mask words the pairs address (max offset + words), tests = sum of |group| over pairs. */
+int dpc_batch_plan_info(const dpc_batch_plan* plan, uint64_t* mask_words, uint64_t* tests, uint64_t* work_items);
+/* d_vps: n_frusta x 16 floats in device memory (update it in place between graph replays). */
+int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
+                   const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks, void* stream);
+/* The per-view form of the same work: one launch per frustum over its groups. */
+int dpc_cull_per_view(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b,
+                      const uint32_t* d_xform, const float* d_planes, size_t ld, const float* h_vps,
+                      uint64_t* d_masks, void* stream);
+
+/* ---- transform tree propagation ----------------------------------------------
+ * dp::transform::Tree::compute (dp/transform/src/Tree.cpp:133-166) on a tree
+ * laid out level-major: the entries of level l occupy slots
+ * [h_level_offsets[l], h_level_offsets[l+1]) in the reference's list order,
+ * and d_parent_slot[s] is the slot of entry s's parent.  For each level in
+ * order, for each slot s of the level:
+ *     if dirty_world[parent] || dirty_local[s]:  W[s] = L[s] * W[parent]; dirty_world[s] = 1
+ * d_dirty_local NULL means every local matrix is dirty (the first compute after
+ * construction).  d_dirty_world (one byte per slot) must be 0 on entry for
+ * every slot the levels write; the caller clears it after consuming the event
+ * (Tree.cpp:162-165).  Entries of one level must not write a slot another entry
+ * of the same level reads (Tree's compile step splits such levels). */
+int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const uint32_t* d_parent_slot,
+                       const uint64_t* h_level_offsets, size_t n_levels, const uint8_t* d_dirty_local,
+                       uint8_t* d_dirty_world, void* stream);
+/* Packs one byte per slot into BitArray words through a slot->index map:
+ * bit d_index_of_slot[s] of d_bits = d_bytes[s] != 0 (d_bits must be cleared first).
+ * Produces the EventWorldMatricesChanged bitset (dp/transform/Tree.h:48-60). */
+int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of_slot, size_t n_slots,
+                              uint32_t* d_bits, void* stream);
+
+/* ---- result diff (ResultBitSet::updateChanged, dp/culling/src/ResultBitSet.cpp:61-108) ---
+ * changed = old ^ new over n bits; the indices of changed bits are written to
+ * d_changed_idx in ascending order (BitArray::traverseBits order) and their
+ * count to *d_count; then old = new if update_old != 0.  d_workspace needs
+ * dpc_result_diff_workspace(n) bytes. */
+size_t dpc_result_diff_workspace(size_t n_bits);
+int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx,
+                    uint32_t* d_count, int update_old, void* d_workspace, void* stream);
+
+/* ---- group bounding box (ManagerBitSet::calculateBoundingBox, dp/culling/src/ManagerBitSet.cpp:268-306)
+ * World-space AABB of every object's eight transformed corners; d_out6 =
+ * lower.xyz, upper.xyz as the reference's Box3f(min, max) returns them.
+ * d_workspace needs dpc_bbox_workspace(n) bytes. */
+size_t dpc_bbox_workspace(size_t n_objects);
+int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+             const float* d_planes, size_t ld, float* d_out6, void* d_workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPCULL_HIP_H */

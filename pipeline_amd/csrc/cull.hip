@@ -1,0 +1,465 @@
+// cull.hip -- single-frustum culling, layout conversion, matrix deltas,
+// group bounding box and result diff.
+//
+// Reference path: dp::culling::cpu::ManagerImpl::cull
+// (dp/culling/cpu/src/ManagerImpl.cpp:467-518), GroupCPU::updateOBBs (:114-162),
+// ManagerBitSet::calculateBoundingBox (dp/culling/src/ManagerBitSet.cpp:268-306)
+// and ResultBitSet::updateChanged (dp/culling/src/ResultBitSet.cpp:61-108).
+#include <cfloat>
+
+#include "cull_math.hpp"
+#include "internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace dpc {
+namespace {
+
+constexpr unsigned kCullThreads = 256;
+
+// One object per lane; one 64-bit mask word per wave from a ballot.
+// The object's four matrix rows come from four row planes, so every load of
+// the wave is a contiguous 1 KiB (rows) or 1 KiB / 512 B (boxes) segment.
+template <bool Indexed>
+__global__ __launch_bounds__(kCullThreads) void cull_kernel(const float4* __restrict__ box_a,
+                                                            const float2* __restrict__ box_b,
+                                                            const uint32_t* __restrict__ xform, size_t n,
+                                                            const float4* __restrict__ planes, size_t ld, Mat16 vp,
+                                                            unsigned long long* __restrict__ mask)
+{
+  const size_t i = static_cast<size_t>(blockIdx.x) * kCullThreads + threadIdx.x;
+  bool vis = false;
+  if (i < n) {
+    const size_t m = Indexed ? static_cast<size_t>(xform[i]) : i;
+    if (m < ld) {
+      const Obb o = make_obb(box_a[i], box_b[i], planes[m], planes[ld + m], planes[2 * ld + m], planes[3 * ld + m]);
+      vis = obb_visible(o, vp.m);
+    } else {
+      vis = true;  // a matrix index outside the array is never dereferenced; such objects stay visible
+    }
+  }
+  const unsigned long long bits = __ballot(vis);
+  if ((threadIdx.x & (kWave - 1)) == 0 && i < n) mask[i >> 6] = bits;
+}
+
+// Row planes <- AoS matrices with a byte stride.  One thread per (matrix, row).
+__global__ void from_aos_kernel(const char* __restrict__ aos, size_t stride, size_t count, float4* __restrict__ planes,
+                                size_t ld)
+{
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count * 4) return;
+  const size_t k = t >> 2, r = t & 3;
+  const float* src = reinterpret_cast<const float*>(aos + k * stride) + 4 * r;
+  planes[r * ld + k] = make_float4(src[0], src[1], src[2], src[3]);
+}
+
+__global__ void to_aos_kernel(const float4* __restrict__ planes, size_t ld, const uint32_t* __restrict__ slot,
+                              size_t count, char* __restrict__ aos, size_t stride)
+{
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count * 4) return;
+  const size_t k = t >> 2, r = t & 3;
+  const size_t s = slot ? static_cast<size_t>(slot[k]) : k;
+  float* dst = reinterpret_cast<float*>(aos + k * stride) + 4 * r;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (s < ld) v = planes[r * ld + s];
+  dst[0] = v.x;
+  dst[1] = v.y;
+  dst[2] = v.z;
+  dst[3] = v.w;
+}
+
+__global__ void scatter_kernel(float4* __restrict__ planes, size_t ld, const uint32_t* __restrict__ idx,
+                               const float4* __restrict__ src, size_t count)
+{
+  const size_t t = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= count * 4) return;
+  const size_t k = t >> 2, r = t & 3;
+  const size_t s = idx[k];
+  if (s < ld) planes[r * ld + s] = src[k * 4 + r];
+}
+
+__global__ void boxes_kernel(const float* __restrict__ lu, size_t n, float4* __restrict__ box_a,
+                             float2* __restrict__ box_b)
+{
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* b = lu + 6 * i;
+  // ManagerBitSet.cpp:102-103: lowerLeft = (lower, 1), extent = (upper - lower, 0).
+  box_a[i] = make_float4(b[0], b[1], b[2], b[3] - b[0]);
+  box_b[i] = make_float2(b[4] - b[1], b[5] - b[2]);
+}
+
+// ---- group bounding box -----------------------------------------------------
+// Box4f::update (dp/math/Boxnt.h:245-258) keeps the running value unless the new
+// point is strictly smaller (lower) / larger (upper).  That is "leftmost extremum"
+// selection, which is associative over ordered ranges: combining a range with the
+// range that follows it, left operand first, reproduces the sequential scan bit for
+// bit (including which of +0/-0 survives and that NaN never enters).
+struct Box6 {
+  float lo[3], hi[3];
+};
+
+__device__ __forceinline__ void box_take(Box6& acc, const float4& p)
+{
+  if (acc.lo[0] > p.x) acc.lo[0] = p.x;
+  if (acc.hi[0] < p.x) acc.hi[0] = p.x;
+  if (acc.lo[1] > p.y) acc.lo[1] = p.y;
+  if (acc.hi[1] < p.y) acc.hi[1] = p.y;
+  if (acc.lo[2] > p.z) acc.lo[2] = p.z;
+  if (acc.hi[2] < p.z) acc.hi[2] = p.z;
+}
+
+__device__ __forceinline__ void box_combine(Box6& left, const Box6& right)
+{
+  for (int c = 0; c < 3; ++c) {
+    if (left.lo[c] > right.lo[c]) left.lo[c] = right.lo[c];
+    if (left.hi[c] < right.hi[c]) left.hi[c] = right.hi[c];
+  }
+}
+
+__device__ __forceinline__ Box6 box_empty()
+{
+  Box6 b;
+  for (int c = 0; c < 3; ++c) {
+    b.lo[c] = FLT_MAX;
+    b.hi[c] = -FLT_MAX;
+  }
+  return b;
+}
+
+// Ordered wave reduction: lane l combines with lane l+off, whose range follows.
+__device__ __forceinline__ Box6 wave_box_reduce(Box6 b)
+{
+  for (int off = 32; off >= 1; off >>= 1) {
+    Box6 r;
+    for (int c = 0; c < 3; ++c) {
+      r.lo[c] = __shfl_down(b.lo[c], off, kWave);
+      r.hi[c] = __shfl_down(b.hi[c], off, kWave);
+    }
+    if ((threadIdx.x & (kWave - 1)) + off < kWave) box_combine(b, r);
+  }
+  return b;
+}
+
+constexpr unsigned kBoxThreads = 256;
+constexpr unsigned kBoxWavesPerBlock = kBoxThreads / kWave;
+
+// Each wave scans one contiguous chunk of objects in 64-object rounds and writes
+// one partial box; partials are stored in wave order.
+__global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4* __restrict__ box_a,
+                                                                   const float2* __restrict__ box_b,
+                                                                   const uint32_t* __restrict__ xform, size_t n,
+                                                                   const float4* __restrict__ planes, size_t ld,
+                                                                   size_t chunk, Box6* __restrict__ partial)
+{
+  const size_t wave = static_cast<size_t>(blockIdx.x) * kBoxWavesPerBlock + (threadIdx.x / kWave);
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const size_t begin = wave * chunk;
+  size_t end = begin + chunk;
+  if (end > n) end = n;
+  Box6 acc = box_empty();
+  for (size_t base = begin; base < end; base += kWave) {
+    const size_t i = base + lane;
+    Box6 mine = box_empty();
+    if (i < end) {
+      const size_t m = xform ? static_cast<size_t>(xform[i]) : i;
+      if (m < ld) {
+        const float4 r0 = planes[m], r1 = planes[ld + m], r2 = planes[2 * ld + m], r3 = planes[3 * ld + m];
+        const float4 a = box_a[i];
+        const float2 b = box_b[i];
+        // ManagerBitSet.cpp:282-294: v0 = lower * M, x/y/z = extent[k] * row k, same add tree as the cull.
+        const Obb o = make_obb(a, b, r0, r1, r2, r3);
+        const float4 v1 = add4(o.p, o.ex);
+        const float4 v2 = add4(o.p, o.ey);
+        const float4 v3 = add4(v1, o.ey);
+        const float4 v4 = add4(o.p, o.ez);
+        const float4 v5 = add4(v1, o.ez);
+        const float4 v6 = add4(v2, o.ez);
+        const float4 v7 = add4(v3, o.ez);
+        box_take(mine, o.p);
+        box_take(mine, v1);
+        box_take(mine, v2);
+        box_take(mine, v3);
+        box_take(mine, v4);
+        box_take(mine, v5);
+        box_take(mine, v6);
+        box_take(mine, v7);
+      }
+    }
+    mine = wave_box_reduce(mine);
+    // lane 0 holds this round's box; fold it into the running box in index order.
+    if (lane == 0) box_combine(acc, mine);
+  }
+  if (lane == 0) partial[wave] = acc;
+}
+
+__global__ __launch_bounds__(kWave) void bbox_final_kernel(const Box6* __restrict__ partial, size_t n_partial,
+                                                           float* __restrict__ out6)
+{
+  const unsigned lane = threadIdx.x;
+  const size_t per = (n_partial + kWave - 1) / kWave;
+  Box6 acc = box_empty();
+  for (size_t k = lane * per; k < (lane + 1) * per && k < n_partial; ++k) box_combine(acc, partial[k]);
+  acc = wave_box_reduce(acc);
+  if (lane == 0) {
+    // return Box3f(lower.xyz, upper.xyz): Boxnt(p0, p1) runs init(), update(p0), update(p1)
+    // (dp/math/Boxnt.h:213-219), which turns the empty box into (-FLT_MAX, +FLT_MAX).
+    Box6 r = box_empty();
+    box_take(r, make_float4(acc.lo[0], acc.lo[1], acc.lo[2], 0.f));
+    box_take(r, make_float4(acc.hi[0], acc.hi[1], acc.hi[2], 0.f));
+    for (int c = 0; c < 3; ++c) {
+      out6[c] = r.lo[c];
+      out6[3 + c] = r.hi[c];
+    }
+  }
+}
+
+size_t bbox_waves(size_t n)
+{
+  // At most 16 waves per CU of partials, at least 64 objects each.
+  size_t waves = (n + kWave - 1) / kWave;
+  const size_t cap = static_cast<size_t>(device_cu_count()) * 16;
+  if (waves > cap) waves = cap;
+  if (waves == 0) waves = 1;
+  waves = (waves + kBoxWavesPerBlock - 1) / kBoxWavesPerBlock * kBoxWavesPerBlock;
+  return waves;
+}
+
+// ---- result diff --------------------------------------------------------------
+constexpr unsigned kDiffThreads = 256;  // one 64-bit word per thread
+
+__device__ __forceinline__ unsigned long long diff_word(const unsigned long long* old_w,
+                                                        const unsigned long long* new_w, size_t w, size_t n_words,
+                                                        size_t n_bits)
+{
+  if (w >= n_words) return 0ull;
+  unsigned long long x = old_w[w] ^ new_w[w];
+  const size_t tail = n_bits & 63;
+  if (w == n_words - 1 && tail) x &= (~0ull) >> (64 - tail);
+  return x;
+}
+
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total)
+{
+  // Wave-level inclusive scan, then scan of the wave totals.
+  const unsigned lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+  uint32_t x = v;
+  for (int off = 1; off < kWave; off <<= 1) {
+    const uint32_t y = __shfl_up(x, off, kWave);
+    if (lane >= static_cast<unsigned>(off)) x += y;
+  }
+  if (lane == kWave - 1) lds[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (unsigned k = 0; k < blockDim.x / kWave; ++k) {
+      const uint32_t t = lds[k];
+      lds[k] = run;
+      run += t;
+    }
+    lds[blockDim.x / kWave] = run;
+  }
+  __syncthreads();
+  const uint32_t r = lds[wave] + x - v;
+  *total = lds[blockDim.x / kWave];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kDiffThreads) void diff_count_kernel(const unsigned long long* __restrict__ old_w,
+                                                                  const unsigned long long* __restrict__ new_w,
+                                                                  size_t n_words, size_t n_bits,
+                                                                  uint32_t* __restrict__ block_count)
+{
+  __shared__ uint32_t lds[kDiffThreads / kWave + 1];
+  const size_t w = static_cast<size_t>(blockIdx.x) * kDiffThreads + threadIdx.x;
+  const uint32_t c = __popcll(diff_word(old_w, new_w, w, n_words, n_bits));
+  uint32_t total;
+  block_exclusive_scan(c, lds, &total);
+  if (threadIdx.x == 0) block_count[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of block counts in place; writes the grand total.
+__global__ __launch_bounds__(kDiffThreads) void diff_scan_kernel(uint32_t* __restrict__ block_count, size_t n_blocks,
+                                                                 uint32_t* __restrict__ d_count)
+{
+  __shared__ uint32_t lds[kDiffThreads / kWave + 1];
+  uint32_t carry = 0;
+  for (size_t base = 0; base < n_blocks; base += kDiffThreads) {
+    const size_t k = base + threadIdx.x;
+    const uint32_t v = k < n_blocks ? block_count[k] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan(v, lds, &total);
+    if (k < n_blocks) block_count[k] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *d_count = carry;
+}
+
+__global__ __launch_bounds__(kDiffThreads) void diff_write_kernel(unsigned long long* __restrict__ old_w,
+                                                                  const unsigned long long* __restrict__ new_w,
+                                                                  size_t n_words, size_t n_bits,
+                                                                  const uint32_t* __restrict__ block_offset,
+                                                                  uint32_t* __restrict__ out, int update_old)
+{
+  __shared__ uint32_t lds[kDiffThreads / kWave + 1];
+  const size_t w = static_cast<size_t>(blockIdx.x) * kDiffThreads + threadIdx.x;
+  unsigned long long x = diff_word(old_w, new_w, w, n_words, n_bits);
+  uint32_t total;
+  uint32_t pos = block_offset[blockIdx.x] + block_exclusive_scan(__popcll(x), lds, &total);
+  while (x) {
+    const unsigned b = __ffsll(static_cast<long long>(x)) - 1;
+    out[pos++] = static_cast<uint32_t>(w * 64 + b);
+    x &= x - 1;
+  }
+  if (update_old && w < n_words) old_w[w] = new_w[w];
+}
+
+unsigned grid_for(size_t work, unsigned per_block)
+{
+  return static_cast<unsigned>((work + per_block - 1) / per_block);
+}
+
+}  // namespace
+}  // namespace dpc
+
+using namespace dpc;
+
+extern "C" {
+
+int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+             const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, void* stream)
+{
+  if (n_objects == 0) return DPC_OK;
+  if (!d_box_a || !d_box_b || !d_planes || !d_mask || !vp)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: null pointer argument");
+  if (!aligned(d_box_a, 16) || !aligned(d_planes, 16) || !aligned(d_box_b, 8) || !aligned(d_mask, 8))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: misaligned device pointer");
+  if (!d_xform && ld < n_objects)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: %zu objects without indices need ld >= n (ld=%zu)",
+                     n_objects, ld);
+  Mat16 m;
+  for (int k = 0; k < 16; ++k) m.m[k] = vp[k];
+  const dim3 grid(grid_for(n_objects, kCullThreads));
+  const auto* a = reinterpret_cast<const float4*>(d_box_a);
+  const auto* b = reinterpret_cast<const float2*>(d_box_b);
+  const auto* p = reinterpret_cast<const float4*>(d_planes);
+  auto* out = reinterpret_cast<unsigned long long*>(d_mask);
+  if (d_xform)
+    hipLaunchKernelGGL(cull_kernel<true>, grid, dim3(kCullThreads), 0, as_stream(stream), a, b, d_xform, n_objects, p,
+                       ld, m, out);
+  else
+    hipLaunchKernelGGL(cull_kernel<false>, grid, dim3(kCullThreads), 0, as_stream(stream), a, b, d_xform, n_objects, p,
+                       ld, m, out);
+  return check_launch("cull_kernel");
+}
+
+int dpc_matrices_from_aos(const void* d_aos, size_t stride_bytes, size_t count, float* d_planes, size_t ld,
+                          void* stream)
+{
+  if (count == 0) return DPC_OK;
+  if (!d_aos || !d_planes) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_from_aos: null pointer");
+  if (stride_bytes < 64 || stride_bytes % 4 || !aligned(d_aos, 4) || !aligned(d_planes, 16) || ld < count)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_from_aos: bad stride/alignment/ld");
+  hipLaunchKernelGGL(from_aos_kernel, dim3(grid_for(count * 4, 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const char*>(d_aos), stride_bytes, count, reinterpret_cast<float4*>(d_planes), ld);
+  return check_launch("from_aos_kernel");
+}
+
+int dpc_matrices_to_aos(const float* d_planes, size_t ld, const uint32_t* d_slot, size_t count, void* d_aos,
+                        size_t stride_bytes, void* stream)
+{
+  if (count == 0) return DPC_OK;
+  if (!d_aos || !d_planes) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_to_aos: null pointer");
+  if (stride_bytes < 64 || stride_bytes % 4 || !aligned(d_aos, 4) || !aligned(d_planes, 16))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_to_aos: bad stride/alignment");
+  if (!d_slot && ld < count) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_to_aos: ld < count");
+  hipLaunchKernelGGL(to_aos_kernel, dim3(grid_for(count * 4, 256)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<const float4*>(d_planes), ld, d_slot, count, static_cast<char*>(d_aos),
+                     stride_bytes);
+  return check_launch("to_aos_kernel");
+}
+
+int dpc_matrices_scatter(float* d_planes, size_t ld, const uint32_t* d_idx, const float* d_src, size_t count,
+                         void* stream)
+{
+  if (count == 0) return DPC_OK;
+  if (!d_planes || !d_idx || !d_src) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_scatter: null pointer");
+  if (!aligned(d_planes, 16) || !aligned(d_src, 16))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_matrices_scatter: misaligned pointer");
+  hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(count * 4, 256)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<float4*>(d_planes), ld, d_idx, reinterpret_cast<const float4*>(d_src), count);
+  return check_launch("scatter_kernel");
+}
+
+int dpc_boxes_from_lower_upper(const float* d_lower_upper, size_t n, float* d_box_a, float* d_box_b, void* stream)
+{
+  if (n == 0) return DPC_OK;
+  if (!d_lower_upper || !d_box_a || !d_box_b || !aligned(d_box_a, 16) || !aligned(d_box_b, 8))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_boxes_from_lower_upper: bad pointer");
+  hipLaunchKernelGGL(boxes_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), d_lower_upper, n,
+                     reinterpret_cast<float4*>(d_box_a), reinterpret_cast<float2*>(d_box_b));
+  return check_launch("boxes_kernel");
+}
+
+size_t dpc_bbox_workspace(size_t n_objects) { return bbox_waves(n_objects) * sizeof(Box6); }
+
+int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+             const float* d_planes, size_t ld, float* d_out6, void* d_workspace, void* stream)
+{
+  if (!d_out6 || !d_workspace) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bbox: null output/workspace");
+  if (n_objects && (!d_box_a || !d_box_b || !d_planes || !aligned(d_box_a, 16) || !aligned(d_box_b, 8) ||
+                    !aligned(d_planes, 16)))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bbox: bad object/matrix pointer");
+  if (n_objects && !d_xform && ld < n_objects)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bbox: objects without indices need ld >= n");
+  const size_t waves = bbox_waves(n_objects);
+  size_t chunk = (n_objects + waves - 1) / waves;
+  chunk = (chunk + kWave - 1) / kWave * kWave;
+  if (chunk == 0) chunk = kWave;
+  auto* partial = static_cast<Box6*>(d_workspace);
+  hipLaunchKernelGGL(bbox_partial_kernel, dim3(static_cast<unsigned>(waves / kBoxWavesPerBlock)), dim3(kBoxThreads), 0,
+                     as_stream(stream), reinterpret_cast<const float4*>(d_box_a),
+                     reinterpret_cast<const float2*>(d_box_b), d_xform, n_objects,
+                     reinterpret_cast<const float4*>(d_planes), ld, chunk, partial);
+  int rc = check_launch("bbox_partial_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kWave), 0, as_stream(stream), partial, waves, d_out6);
+  return check_launch("bbox_final_kernel");
+}
+
+size_t dpc_result_diff_workspace(size_t n_bits)
+{
+  const size_t n_words = (n_bits + 63) / 64;
+  const size_t blocks = (n_words + kDiffThreads - 1) / kDiffThreads;
+  return (blocks ? blocks : 1) * sizeof(uint32_t);
+}
+
+int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx, uint32_t* d_count,
+                    int update_old, void* d_workspace, void* stream)
+{
+  if (!d_count || !d_workspace) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: null count/workspace");
+  if (n_bits == 0) return dpc_fill_u32(d_count, 0u, 1, stream);
+  if (!d_old || !d_new || !d_changed_idx || !aligned(d_old, 8) || !aligned(d_new, 8))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: bad mask pointer");
+  if (n_bits > 0xFFFFFFFFull) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: more than 2^32 bits");
+  const size_t n_words = (n_bits + 63) / 64;
+  const unsigned blocks = grid_for(n_words, kDiffThreads);
+  auto* counts = static_cast<uint32_t*>(d_workspace);
+  auto* o = reinterpret_cast<unsigned long long*>(d_old);
+  auto* nw = reinterpret_cast<const unsigned long long*>(d_new);
+  hipLaunchKernelGGL(diff_count_kernel, dim3(blocks), dim3(kDiffThreads), 0, as_stream(stream), o, nw, n_words, n_bits,
+                     counts);
+  int rc = check_launch("diff_count_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(diff_scan_kernel, dim3(1), dim3(kDiffThreads), 0, as_stream(stream), counts,
+                     static_cast<size_t>(blocks), d_count);
+  rc = check_launch("diff_scan_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(diff_write_kernel, dim3(blocks), dim3(kDiffThreads), 0, as_stream(stream), o, nw, n_words, n_bits,
+                     counts, d_changed_idx, update_old);
+  return check_launch("diff_write_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// cull_math.hpp -- device-side FP32 math of the culling path, in the exact
+// evaluation order of nvpro-pipeline's dp::math and dp::culling::cpu code.
+//
+// Every function is a sequence of separate IEEE binary32 multiplies and adds.
+// The translation units that include this header are compiled with
+// -ffp-contract=off and the pragma below, so no multiply-add is fused and
+// nothing is reassociated; float denormals are preserved
+// (-fno-gpu-flush-denormals-to-zero).  With that, the device produces the
+// same bits as the reference's scalar C++ path.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
+
+namespace dpc {
+
+// A 4x4 row-major matrix passed by value in kernel arguments: the compiler
+// loads it into scalar registers once per wave, and the VALU multiplies read
+// it from there.
+struct Mat16 {
+  float m[16];
+};
+
+__device__ __forceinline__ float4 add4(float4 a, float4 b)
+{
+  return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+__device__ __forceinline__ float4 scale4(float s, float4 v)
+{
+  return make_float4(v.x * s, v.y * s, v.z * s, v.w * s);
+}
+
+// v * M, dp/math/Matmnt.h:1371-1379: out[c] = ((v0*M0c + v1*M1c) + v2*M2c) + v3*M3c.
+__device__ __forceinline__ float4 vecmat(float4 v, const float* m)
+{
+  float4 r;
+  r.x = v.x * m[0];
+  r.x = r.x + v.y * m[4];
+  r.x = r.x + v.z * m[8];
+  r.x = r.x + v.w * m[12];
+  r.y = v.x * m[1];
+  r.y = r.y + v.y * m[5];
+  r.y = r.y + v.z * m[9];
+  r.y = r.y + v.w * m[13];
+  r.z = v.x * m[2];
+  r.z = r.z + v.y * m[6];
+  r.z = r.z + v.z * m[10];
+  r.z = r.z + v.w * m[14];
+  r.w = v.x * m[3];
+  r.w = r.w + v.y * m[7];
+  r.w = r.w + v.z * m[11];
+  r.w = r.w + v.w * m[15];
+  return r;
+}
+
+// Row i of A*B given row i of A and the four rows of B
+// (dp/math/Matmnt.h:1381-1417: ((a_i0*B0 + a_i1*B1) + a_i2*B2) + a_i3*B3, per column).
+__device__ __forceinline__ float4 rowmat(float4 a, float4 b0, float4 b1, float4 b2, float4 b3)
+{
+  float4 r;
+  r.x = a.x * b0.x;
+  r.x = r.x + a.y * b1.x;
+  r.x = r.x + a.z * b2.x;
+  r.x = r.x + a.w * b3.x;
+  r.y = a.x * b0.y;
+  r.y = r.y + a.y * b1.y;
+  r.y = r.y + a.z * b2.y;
+  r.y = r.y + a.w * b3.y;
+  r.z = a.x * b0.z;
+  r.z = r.z + a.y * b1.z;
+  r.z = r.z + a.z * b2.z;
+  r.z = r.z + a.w * b3.z;
+  r.w = a.x * b0.w;
+  r.w = r.w + a.y * b1.w;
+  r.w = r.w + a.z * b2.w;
+  r.w = r.w + a.w * b3.w;
+  return r;
+}
+
+// Object-space box transformed to world space:
+// GroupCPU::updateOBBs (dp/culling/cpu/src/ManagerImpl.cpp:131, 152-156).
+struct Obb {
+  float4 p, ex, ey, ez;
+};
+
+// lower = (lo.xyz, 1) and extent = (ex.xyz, 0), as ManagerBitSet::objectSetBoundingBox
+// stores them (dp/culling/src/ManagerBitSet.cpp:99-109); r0..r3 are the rows of the
+// object's matrix.
+__device__ __forceinline__ Obb make_obb(float4 box_a, float2 box_b, float4 r0, float4 r1, float4 r2, float4 r3)
+{
+  Obb o;
+  const float lx = box_a.x, ly = box_a.y, lz = box_a.z;
+  // point = (lx, ly, lz, 1) * M, left to right.
+  o.p.x = lx * r0.x;
+  o.p.x = o.p.x + ly * r1.x;
+  o.p.x = o.p.x + lz * r2.x;
+  o.p.x = o.p.x + 1.0f * r3.x;
+  o.p.y = lx * r0.y;
+  o.p.y = o.p.y + ly * r1.y;
+  o.p.y = o.p.y + lz * r2.y;
+  o.p.y = o.p.y + 1.0f * r3.y;
+  o.p.z = lx * r0.z;
+  o.p.z = o.p.z + ly * r1.z;
+  o.p.z = o.p.z + lz * r2.z;
+  o.p.z = o.p.z + 1.0f * r3.z;
+  o.p.w = lx * r0.w;
+  o.p.w = o.p.w + ly * r1.w;
+  o.p.w = o.p.w + lz * r2.w;
+  o.p.w = o.p.w + 1.0f * r3.w;
+  o.ex = scale4(box_a.w, r0);
+  o.ey = scale4(box_b.x, r1);
+  o.ez = scale4(box_b.y, r2);
+  return o;
+}
+
+// determineCullFlags (ManagerImpl.cpp:199-229) folded into the AND over corners:
+// a[k] accumulates "every corner so far carries flag bit k".  The else-if of the
+// reference is kept: the "high" flag of an axis is set only when the "low" one is not.
+struct PlaneAnd {
+  bool a0, a1, a2, a3, a4, a5;
+};
+
+__device__ __forceinline__ void corner_flags(float4 c, PlaneAnd& s)
+{
+  const bool lx = c.x <= -c.w;
+  const bool hx = !lx && (c.w <= c.x);
+  const bool ly = c.y <= -c.w;
+  const bool hy = !ly && (c.w <= c.y);
+  const bool lz = c.z <= -c.w;
+  const bool hz = !lz && (c.w <= c.z);
+  s.a0 = s.a0 && lx;
+  s.a1 = s.a1 && hx;
+  s.a2 = s.a2 && ly;
+  s.a3 = s.a3 && hy;
+  s.a4 = s.a4 && lz;
+  s.a5 = s.a5 && hz;
+}
+
+// isVisible(projection, OBB) (ManagerImpl.cpp:263-289).  The reference returns
+// !cfo || !cfa; since cfa is the AND and cfo the OR of the same eight flag words,
+// cfo == 0 implies cfa == 0, so the result is cfa == 0.
+__device__ __forceinline__ bool obb_visible(const Obb& o, const float* vp)
+{
+  const float4 v0 = vecmat(o.p, vp);
+  const float4 x = vecmat(o.ex, vp);
+  const float4 y = vecmat(o.ey, vp);
+  const float4 z = vecmat(o.ez, vp);
+  const float4 c1 = add4(v0, x);
+  const float4 c2 = add4(v0, y);
+  const float4 c3 = add4(c1, y);
+  const float4 c4 = add4(v0, z);
+  const float4 c5 = add4(c1, z);
+  const float4 c6 = add4(c2, z);
+  const float4 c7 = add4(c3, z);
+  PlaneAnd s{true, true, true, true, true, true};
+  corner_flags(v0, s);
+  corner_flags(c1, s);
+  corner_flags(c2, s);
+  corner_flags(c3, s);
+  corner_flags(c4, s);
+  corner_flags(c5, s);
+  corner_flags(c6, s);
+  corner_flags(c7, s);
+  return !(s.a0 || s.a1 || s.a2 || s.a3 || s.a4 || s.a5);
+}
+
+}  // namespace dpc

@@ -1,0 +1,246 @@
+// runtime.hip -- device memory, streams, events, graphs and small utility
+// kernels behind the C ABI.
+#include <mutex>
+#include <vector>
+
+#include "internal.hpp"
+
+namespace dpc {
+
+namespace {
+thread_local std::string g_last_error;
+std::mutex g_cu_mutex;
+std::vector<int> g_cu_count;  // per device, 0 = unknown
+}  // namespace
+
+int set_error(int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int device_cu_count()
+{
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lock(g_cu_mutex);
+  if (static_cast<size_t>(dev) >= g_cu_count.size()) g_cu_count.resize(dev + 1, 0);
+  if (g_cu_count[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    g_cu_count[dev] = cus;
+  }
+  return g_cu_count[dev];
+}
+
+namespace {
+__global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value, size_t n)
+{
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = value;
+}
+}  // namespace
+
+}  // namespace dpc
+
+using namespace dpc;
+
+extern "C" {
+
+int dpc_abi_version(void) { return DPC_ABI_VERSION; }
+
+const char* dpc_last_error(void) { return g_last_error.c_str(); }
+
+int dpc_device_info(int device, char* name, size_t name_len, int* cu_count, int* clock_khz, size_t* total_mem,
+                    int* l2_bytes)
+{
+  hipDeviceProp_t prop;
+  DPC_HIP_TRY(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  if (name && name_len) {
+    snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (clock_khz) *clock_khz = prop.clockRate;
+  if (total_mem) *total_mem = prop.totalGlobalMem;
+  if (l2_bytes) *l2_bytes = prop.l2CacheSize;
+  return DPC_OK;
+}
+
+int dpc_set_device(int device)
+{
+  DPC_HIP_TRY(hipSetDevice(device), "hipSetDevice");
+  return DPC_OK;
+}
+
+int dpc_malloc(void** d_ptr, size_t bytes)
+{
+  if (!d_ptr) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_malloc: null output pointer");
+  *d_ptr = nullptr;
+  if (bytes == 0) return DPC_OK;
+  hipError_t e = hipMalloc(d_ptr, bytes);
+  if (e == hipErrorOutOfMemory) return set_error(DPC_ERR_OUT_OF_MEMORY, "dpc_malloc: out of memory (%zu bytes)", bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  return DPC_OK;
+}
+
+int dpc_free(void* d_ptr)
+{
+  if (!d_ptr) return DPC_OK;
+  DPC_HIP_TRY(hipFree(d_ptr), "hipFree");
+  return DPC_OK;
+}
+
+int dpc_host_alloc(void** h_ptr, size_t bytes)
+{
+  if (!h_ptr) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_host_alloc: null output pointer");
+  *h_ptr = nullptr;
+  if (bytes == 0) return DPC_OK;
+  DPC_HIP_TRY(hipHostMalloc(h_ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+  return DPC_OK;
+}
+
+int dpc_host_free(void* h_ptr)
+{
+  if (!h_ptr) return DPC_OK;
+  DPC_HIP_TRY(hipHostFree(h_ptr), "hipHostFree");
+  return DPC_OK;
+}
+
+int dpc_memcpy_h2d(void* d_dst, const void* src, size_t bytes, void* stream)
+{
+  if (bytes == 0) return DPC_OK;
+  DPC_HIP_TRY(hipMemcpyAsync(d_dst, src, bytes, hipMemcpyHostToDevice, as_stream(stream)), "hipMemcpyAsync H2D");
+  return DPC_OK;
+}
+
+int dpc_memcpy_d2h(void* dst, const void* d_src, size_t bytes, void* stream)
+{
+  if (bytes == 0) return DPC_OK;
+  DPC_HIP_TRY(hipMemcpyAsync(dst, d_src, bytes, hipMemcpyDeviceToHost, as_stream(stream)), "hipMemcpyAsync D2H");
+  return DPC_OK;
+}
+
+int dpc_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream)
+{
+  if (bytes == 0) return DPC_OK;
+  DPC_HIP_TRY(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)), "hipMemcpyAsync D2D");
+  return DPC_OK;
+}
+
+int dpc_fill_u32(uint32_t* d_dst, uint32_t value, size_t n, void* stream)
+{
+  if (n == 0) return DPC_OK;
+  if (!d_dst || !aligned(d_dst, 4)) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_fill_u32: bad destination");
+  const unsigned threads = 256;
+  size_t blocks = (n + threads - 1) / threads;
+  const size_t cap = static_cast<size_t>(device_cu_count()) * 16;
+  if (blocks > cap) blocks = cap;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, as_stream(stream), d_dst,
+                     value, n);
+  return check_launch("fill_u32_kernel");
+}
+
+int dpc_stream_create(void** stream)
+{
+  if (!stream) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_stream_create: null output");
+  hipStream_t s = nullptr;
+  // Default priority, non-blocking with respect to the null stream.
+  DPC_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  *stream = s;
+  return DPC_OK;
+}
+
+int dpc_stream_destroy(void* stream)
+{
+  if (!stream) return DPC_OK;
+  DPC_HIP_TRY(hipStreamDestroy(as_stream(stream)), "hipStreamDestroy");
+  return DPC_OK;
+}
+
+int dpc_stream_sync(void* stream)
+{
+  DPC_HIP_TRY(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+  return DPC_OK;
+}
+
+int dpc_device_sync(void)
+{
+  DPC_HIP_TRY(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  return DPC_OK;
+}
+
+int dpc_event_create(void** event)
+{
+  if (!event) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_event_create: null output");
+  hipEvent_t e = nullptr;
+  DPC_HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+  *event = e;
+  return DPC_OK;
+}
+
+int dpc_event_destroy(void* event)
+{
+  if (!event) return DPC_OK;
+  DPC_HIP_TRY(hipEventDestroy(static_cast<hipEvent_t>(event)), "hipEventDestroy");
+  return DPC_OK;
+}
+
+int dpc_event_record(void* event, void* stream)
+{
+  DPC_HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(event), as_stream(stream)), "hipEventRecord");
+  return DPC_OK;
+}
+
+int dpc_event_sync(void* event)
+{
+  DPC_HIP_TRY(hipEventSynchronize(static_cast<hipEvent_t>(event)), "hipEventSynchronize");
+  return DPC_OK;
+}
+
+int dpc_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+  if (!ms) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_event_elapsed_ms: null output");
+  DPC_HIP_TRY(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)),
+              "hipEventElapsedTime");
+  return DPC_OK;
+}
+
+int dpc_graph_begin(void* stream)
+{
+  if (!stream) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_graph_begin: capture needs a created stream");
+  DPC_HIP_TRY(hipStreamBeginCapture(as_stream(stream), hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  return DPC_OK;
+}
+
+int dpc_graph_end(void* stream, void** graph_exec)
+{
+  if (!graph_exec) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_graph_end: null output");
+  hipGraph_t g = nullptr;
+  DPC_HIP_TRY(hipStreamEndCapture(as_stream(stream), &g), "hipStreamEndCapture");
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+  *graph_exec = ge;
+  return DPC_OK;
+}
+
+int dpc_graph_launch(void* graph_exec, void* stream)
+{
+  DPC_HIP_TRY(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec), as_stream(stream)), "hipGraphLaunch");
+  return DPC_OK;
+}
+
+int dpc_graph_destroy(void* graph_exec)
+{
+  if (!graph_exec) return DPC_OK;
+  DPC_HIP_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)), "hipGraphExecDestroy");
+  return DPC_OK;
+}
+
+}  // extern "C"

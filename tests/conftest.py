@@ -1,0 +1,50 @@
+import os
+import subprocess
+import sys
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    # The CPU oracle is a few hundred lines of C; build it when it is missing
+    # (it is test infrastructure, compiled with gcc -ffp-contract=off).
+    if not (ROOT / "oracle" / "liboracle.so").exists():
+        subprocess.run(["make", "-s", "-C", str(ROOT / "oracle"), "liboracle.so"], check=True)
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+    oracle.load()
+    return oracle
+
+
+@pytest.fixture(scope="session")
+def ref():
+    from oracle import oracle
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref (the compiled reference) is not built here")
+    return oracle.load_ref()
+
+
+@pytest.fixture(scope="session")
+def lib():
+    from pipeline_amd import _lib
+    return _lib.load()
+
+
+@pytest.fixture(autouse=True)
+def _announce_gpu_case(request):
+    # Print the case name before each GPU test runs, so a run that ends the box
+    # still shows which case was running.
+    if request.node.get_closest_marker("gpu") is not None:
+        sys.stdout.write(f"\n[gpu case] {request.node.nodeid}\n")
+        sys.stdout.flush()
+    yield

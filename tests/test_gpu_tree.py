@@ -1,0 +1,169 @@
+"""GPU parity of level-synchronous world-matrix propagation (dpc_tree_propagate)
+with the oracle of dp::transform::Tree::compute.  World matrices bit-exact
+(0 ulp, well inside the 2-ulp bound), plus the FP64 error report; then the cull
+over the propagated matrices, bit-exact."""
+import numpy as np
+import pytest
+
+from gpu_util import mismatches
+from pipeline_amd import _lib
+from pipeline_amd import device as dev
+from pipeline_amd import scenes
+
+pytestmark = pytest.mark.gpu
+F = np.float32
+
+
+def ulp_diff(a, b):
+    ia = np.ascontiguousarray(a, F).view(np.int32).astype(np.int64)
+    ib = np.ascontiguousarray(b, F).view(np.int32).astype(np.int64)
+    ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
+    ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
+    return np.abs(ia - ib)
+
+
+class DeviceTree:
+    def __init__(self, ts: scenes.TreeScene):
+        self.ts = ts
+        self.n = ts.n_slots
+        self.local = dev.DeviceArray.from_host(dev.matrices_to_planes(ts.local))
+        world0 = np.zeros_like(ts.local)
+        world0[0] = np.eye(4, dtype=F)
+        self.world = dev.DeviceArray.from_host(dev.matrices_to_planes(world0))
+        self.parent = dev.DeviceArray.from_host(ts.parent_slot)
+        self.dirty_world = dev.DeviceArray(((self.n + 3) // 4) * 4, np.uint8)
+        self.dirty_world.fill_u32(0)
+        self.offsets = np.ascontiguousarray(ts.level_offsets, np.uint64)
+
+    def propagate(self, dirty_local=None, stream=None):
+        import ctypes as C
+        lib = _lib.load()
+        d_dl = None
+        if dirty_local is not None:
+            d_dl = dev.DeviceArray.from_host(dirty_local)
+        off = (C.c_uint64 * self.offsets.size)(*self.offsets.tolist())
+        _lib.check(lib.dpc_tree_propagate(self.local.ptr, self.world.ptr, self.n, self.parent.ptr, off,
+                                          self.offsets.size - 1, d_dl.ptr if d_dl else None, self.dirty_world.ptr,
+                                          stream))
+        _lib.check(lib.dpc_stream_sync(stream))
+        if d_dl:
+            d_dl.free()
+
+    def world_by_slot(self):
+        return np.transpose(self.world.download(), (1, 0, 2))
+
+    def free(self):
+        for a in (self.local, self.world, self.parent, self.dirty_world):
+            a.free()
+
+
+def oracle_world(orc, ts):
+    """Tree::compute on the reference's index-addressed arrays and level lists."""
+    offsets, entries = ts.reference_levels()
+    local = np.ascontiguousarray(ts.local_by_index().reshape(-1, 16))
+    world = np.zeros_like(local)
+    world[0] = np.eye(4, dtype=F).reshape(16)
+    dirty_local = np.ones(ts.n_slots, np.uint8)
+    dirty_world = np.zeros(ts.n_slots, np.uint8)
+    event = orc.tree_compute(offsets, entries, local, world, dirty_local, dirty_world)
+    return world.reshape(-1, 4, 4), event
+
+
+@pytest.mark.parametrize("n_nodes,seed", [(37, 1), (2_000, 5), (50_000, 8)])
+def test_propagate_matches_oracle(orc, n_nodes, seed):
+    ts = scenes.tree_scene(n_nodes, seed)
+    t = DeviceTree(ts)
+    t.propagate()
+    got = t.world_by_slot()
+    want, event = oracle_world(orc, ts)
+    want_by_slot = want[ts.index_of_slot]
+    assert np.array_equal(got[1:].view(np.uint32), want_by_slot[1:].view(np.uint32))
+    dw = t.dirty_world.download()[: ts.n_slots]
+    assert np.array_equal(dw[1:], event[ts.index_of_slot][1:])
+    t.free()
+
+
+def test_config2_tree_1m_and_cull(orc):
+    """1,000,000 nodes (depth <= 16, geometric children), then cull of ~2.4M leaf objects."""
+    ts = scenes.tree_scene(1_000_000, 2)
+    t = DeviceTree(ts)
+    t.propagate()
+    got = t.world_by_slot()
+    want, _ = oracle_world(orc, ts)
+    want_by_slot = want[ts.index_of_slot]
+    ulps = ulp_diff(got[1:], want_by_slot[1:])
+    # FP64 oracle of the same recurrence: max relative error of the FP32 result
+    offsets, entries = ts.reference_levels()
+    w64 = orc.tree_compute_f64(offsets, entries, ts.local_by_index().astype(np.float64))[ts.index_of_slot]
+    scale = np.abs(w64[1:]).max(axis=(1, 2), keepdims=True)
+    rel = float((np.abs(got[1:].astype(np.float64) - w64[1:]) / scale).max())
+    print(f"config2: {ts.n_slots - 1} nodes, {len(ts.level_offsets) - 1} levels, max ulp vs FP32 oracle "
+          f"{int(ulps.max())}, max relative error vs FP64 {rel:.3e}")
+    assert ulps.max() == 0
+    assert rel < 1e-4
+
+    # cull the leaf objects against the propagated world matrices (by slot)
+    box = dev.upload_scene(ts.obj_lower, ts.obj_upper, np.eye(4, dtype=F)[None], None)
+    box.planes.free()
+    box.planes, box.ld = t.world, ts.n_slots
+    box.xform = dev.DeviceArray.from_host(ts.obj_slot)
+    mask = dev.cull(box, ts.vp).download()
+    l4, e4 = scenes.extents(ts.obj_lower, ts.obj_upper)
+    want_mask = orc.cull_group(l4, e4, ts.obj_slot, want_by_slot, ts.vp)
+    m = ts.obj_slot.shape[0]
+    n_mis = mismatches(mask, want_mask, m)
+    print(f"config2 cull: {m} objects, {int(np.unpackbits(want_mask.view(np.uint8)).sum())} visible, "
+          f"{n_mis} mismatching bits")
+    assert n_mis == 0
+    box.box_a.free()
+    box.box_b.free()
+    box.xform.free()
+    t.free()
+
+
+def test_dirty_propagation(orc):
+    """Second compute with a few dirty locals: only dirty nodes and their
+    descendants are recomputed (Tree.cpp:155-159), and the event lists exactly them."""
+    ts = scenes.tree_scene(20_000, 17)
+    t = DeviceTree(ts)
+    t.propagate()
+    t.dirty_world.fill_u32(0)   # the caller clears after consuming the event (Tree.cpp:164-165)
+    # oracle state after the first compute
+    offsets, entries = ts.reference_levels()
+    local = np.ascontiguousarray(ts.local_by_index().reshape(-1, 16))
+    world = np.zeros_like(local)
+    world[0] = np.eye(4, dtype=F).reshape(16)
+    orc.tree_compute(offsets, entries, local, world, np.ones(ts.n_slots, np.uint8), np.zeros(ts.n_slots, np.uint8))
+    # change some locals (by slot on the device, by index in the oracle)
+    rng = np.random.default_rng(3)
+    slots = rng.choice(np.arange(1, ts.n_slots), 40, replace=False)
+    new = scenes.trs_matrices(rng, 40, scale=(0.8, 1.25), trans=(-10, 10))
+    import ctypes as C
+    lib = _lib.load()
+    d_idx = dev.DeviceArray.from_host(slots.astype(np.uint32))
+    d_new = dev.DeviceArray.from_host(new.reshape(-1, 16))
+    _lib.check(lib.dpc_matrices_scatter(t.local.ptr, t.n, d_idx.ptr, d_new.ptr, 40, None))
+    dirty_slots = np.zeros(ts.n_slots, np.uint8)
+    dirty_slots[slots] = 1
+    t.propagate(dirty_local=dirty_slots)
+    idx = ts.index_of_slot[slots]
+    local[idx] = new.reshape(-1, 16)
+    dl = np.zeros(ts.n_slots, np.uint8)
+    dl[idx] = 1
+    event = orc.tree_compute(offsets, entries, local, world, dl, np.zeros(ts.n_slots, np.uint8))
+    got = t.world_by_slot()
+    want = world.reshape(-1, 4, 4)[ts.index_of_slot]
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    dw = t.dirty_world.download()[: ts.n_slots]
+    assert np.array_equal(dw, event[ts.index_of_slot])
+    assert 40 <= dw.sum() < ts.n_slots // 2
+    # the event as a BitArray over transform indices
+    bits = dev.DeviceArray(((ts.n_slots + 63) // 64) * 2, np.uint32)
+    bits.fill_u32(0)
+    d_map = dev.DeviceArray.from_host(ts.index_of_slot)
+    _lib.check(lib.dpc_bytes_to_bits_indexed(t.dirty_world.ptr, d_map.ptr, ts.n_slots, bits.ptr, None))
+    got_bits = np.unpackbits(bits.download().view(np.uint8), bitorder="little")[: ts.n_slots]
+    assert np.array_equal(got_bits, event)
+    for d in (d_idx, d_new, bits, d_map):
+        d.free()
+    t.free()

@@ -1,0 +1,184 @@
+"""Known-answer cases for the oracle's culling functions.
+
+The reference's culling translation units include Boost headers that this
+image does not have, so they are not compiled here.  These cases pin
+orc_obb_build / orc_is_visible_obb / orc_cull_group / orc_bbox_group instead.
+Each input is made of exactly representable values and its expected result is
+derived by hand in the comments from dp/culling/cpu/src/ManagerImpl.cpp; each
+association case is built so that the plausible alternative evaluation order
+gives the other answer (the test evaluates that alternative too, to show the
+case discriminates).
+"""
+import numpy as np
+import pytest
+
+F = np.float32
+ID = np.eye(4, dtype=F)
+
+
+def obb_of(orc, lower, upper, m):
+    l4 = np.array([*lower, 1], F)
+    e4 = np.array([*(np.array(upper, F) - np.array(lower, F)), 0], F)
+    return orc.obb_build(l4, e4, m)
+
+
+def visible(orc, lower, upper, m, vp):
+    return orc.is_visible_obb(vp, obb_of(orc, lower, upper, m))
+
+
+# ---- identity transforms: clip space = world space, w = 1 --------------------
+@pytest.mark.parametrize("lower,upper,want", [
+    ((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), True),    # inside every plane: cfo = 0
+    ((2, -0.5, -0.5), (3, 0.5, 0.5), False),        # every corner w <= x: cfa = 0x02
+    ((-3, -0.5, -0.5), (-2, 0.5, 0.5), False),      # every corner x <= -w: cfa = 0x01
+    ((-0.5, 1.5, -0.5), (0.5, 2.5, 0.5), False),    # +y: 0x08
+    ((-0.5, -0.5, -9), (0.5, 0.5, -1.5), False),    # -z: 0x10
+    ((0.5, -0.5, -0.5), (1.5, 0.5, 0.5), True),     # straddles x = w
+    ((-5, -5, -5), (5, 5, 5), True),                # contains the frustum: no common flag
+    ((1, -0.5, -0.5), (2, 0.5, 0.5), False),        # corner exactly on x = w: `<=` flags it
+    ((-2, -0.5, -0.5), (-1, 0.5, 0.5), False),      # corner exactly on x = -w
+    ((0.25, 0.25, 0.25), (0.25, 0.25, 0.25), True),  # zero extent inside
+    ((4, 4, 4), (4, 4, 4), False),                  # zero extent outside
+])
+def test_identity_cases(orc, lower, upper, want):
+    assert visible(orc, lower, upper, ID, ID) is want
+
+
+def test_le_not_lt(orc):
+    # Box [1,2] in x touches the plane x = w = 1 with its low corners.  With `<=`
+    # (ManagerImpl.cpp:207) all eight corners carry 0x02 and the box is culled;
+    # a `<` test would leave the x = 1 corners unflagged and keep it visible.
+    o = obb_of(orc, (1, -0.5, -0.5), (2, 0.5, 0.5), ID)
+    c = orc.corners_clip(ID, o)
+    assert np.all(c[:, 3] <= c[:, 0])
+    assert not np.all(c[:, 3] < c[:, 0])
+    assert orc.is_visible_obb(ID, o) is False
+
+
+def test_else_if_flags_behind_eye(orc):
+    # VP = diag(1,1,1,-1): every corner has w = -1, so -w = 1.  Box [0.5,2]^3.
+    # Corners with coordinate 0.5 satisfy c <= -w (low flag) AND w <= c; the
+    # reference's else-if (ManagerImpl.cpp:203-226) sets only the low flag.
+    # Corners with coordinate 2 get only the high flag.  No flag is common to
+    # all eight corners -> cfa = 0 -> visible.  Setting both flags (no else)
+    # would leave the high flag common to all corners -> culled.
+    vp = np.diag([1, 1, 1, -1]).astype(F)
+    o = obb_of(orc, (0.5, 0.5, 0.5), (2, 2, 2), ID)
+    assert orc.is_visible_obb(vp, o) is True
+    c = orc.corners_clip(vp, o)
+    assert np.all(c[:, 3] <= c[:, 0])  # without the else, 0x02 would be common
+
+
+def test_nan_w_is_visible(orc):
+    # A NaN in M[3][3] makes every corner's w NaN: every comparison is false, no
+    # flag is set, cfo = 0 -> visible although the box is far outside.
+    m = ID.copy()
+    m[3, 3] = np.nan
+    assert visible(orc, (10, 10, 10), (11, 11, 11), m, ID) is True
+    assert visible(orc, (10, 10, 10), (11, 11, 11), ID, ID) is False
+
+
+def test_corner_add_tree(orc):
+    """vectors[5] = vectors[1] + z (ManagerImpl.cpp:279), not vectors[4] + x.
+
+    M rows: (-1,0,0,0), (0,1,0,0), (-3*2^-24, 0, 1, 0), (0,0,0,1); box lower
+    (-3,0,0), upper (-1,0.5,1) -> extent (2,0.5,1).  OBB: point = (3,0,0,1),
+    ex = (-2,0,0,0), ey = (0,0.5,0,0), ez = (-3*2^-24,0,1,0).
+    VP = diag(1,1,1,w0) with w0 = 1 - 3*2^-24, so every corner has w = w0.
+    Corner x values: v0 = 3, c1 = 1, c2 = 3, c3 = 1,
+      c4 = 3 - 3*2^-24 -> rounds to 3 - 2^-22 (ulp 2^-22 in [2,4)),
+      c5 = c1 + z = 1 - 3*2^-24 (exact) = w0  -> w <= x holds,
+      c6 = 3 - 2^-22, c7 = c3 + z = w0.
+    All eight corners carry 0x02 -> culled.  The other association,
+    c5 = c4 + x = (3 - 2^-22) - 2 = 1 - 2^-22 < w0, loses the flag -> visible."""
+    e = F(3 * 2.0 ** -24)
+    m = np.array([[-1, 0, 0, 0], [0, 1, 0, 0], [-e, 0, 1, 0], [0, 0, 0, 1]], F)
+    w0 = F(1) - e
+    vp = np.diag([1, 1, 1, w0]).astype(F)
+    o = obb_of(orc, (-3, 0, 0), (-1, 0.5, 1), m)
+    assert np.array_equal(o, np.array([[3, 0, 0, 1], [-2, 0, 0, 0], [0, 0.5, 0, 0], [-e, 0, 1, 0]], F))
+    c = orc.corners_clip(vp, o)
+    assert c[5, 0] == w0 and c[7, 0] == w0 and c[4, 0] == F(3) - F(2.0 ** -22)
+    assert orc.is_visible_obb(vp, o) is False
+    alt_c5 = (F(3) + F(-e)) + F(-2)   # (v0 + z) + x
+    assert alt_c5 < w0
+
+
+def test_two_stage_rounding(orc):
+    """v0 = (lower * M) * VP (ManagerImpl.cpp:131 then :269), never lower * (M * VP).
+
+    s = 1 + 2^-23, t = 1 - 2^-24, w0 = 3 + 2^-22; M = diag(s,s,s,1), VP = diag(t,1,1,w0),
+    zero-extent box at (3,0,0).  Reference: point.x = 3*s = 3 + 1.5*2^-22 -> ties to
+    even -> 3 + 2^-21; v0.x = (3 + 2^-21)*t -> 3 + 2^-22 = w0 -> w <= x -> culled.
+    Pre-multiplied: (M*VP)[0][0] = s*t -> 1 (rounds down), v0.x = 3 < w0 -> visible."""
+    s, t = F(1 + 2.0 ** -23), F(1 - 2.0 ** -24)
+    w0 = F(3 + 2.0 ** -22)
+    m = np.diag([s, s, s, 1]).astype(F)
+    vp = np.diag([t, 1, 1, w0]).astype(F)
+    o = obb_of(orc, (3, 0, 0), (3, 0, 0), m)
+    assert o[0, 0] == F(3 + 2.0 ** -21)
+    assert orc.corners_clip(vp, o)[0, 0] == w0
+    assert orc.is_visible_obb(vp, o) is False
+    mvp = orc.mat44_mul(m, vp)
+    assert mvp[0, 0] == F(1)
+    assert orc.is_visible_obb(mvp, orc.obb_build(np.array([3, 0, 0, 1], F), np.zeros(4, F), ID)) is True
+
+
+def test_cull_group_bits_and_tail(orc):
+    """Mask layout: bit i of 64-bit word i>>6 (BitArray.h:215-233); bits >= n are 0;
+    objects share matrices through their transform index."""
+    n = 70
+    lower = np.zeros((n, 3), F)
+    upper = np.zeros((n, 3), F)
+    vis = np.zeros(n, bool)
+    for i in range(n):
+        if i % 3 == 0:
+            lower[i], upper[i], vis[i] = (-0.5, -0.5, -0.5), (0.5, 0.5, 0.5), True
+        else:
+            lower[i], upper[i] = (2, 2, 2), (3, 3, 3)
+    l4 = np.concatenate([lower, np.ones((n, 1), F)], 1)
+    e4 = np.concatenate([upper - lower, np.zeros((n, 1), F)], 1)
+    mats = np.stack([ID, ID])
+    xform = np.arange(n) % 2
+    mask = orc.cull_group(l4, e4, xform, mats, ID)
+    assert mask.shape == (2,)
+    bits = np.unpackbits(mask.view(np.uint8), bitorder="little")
+    assert np.array_equal(bits[:n].astype(bool), vis)
+    assert not bits[n:].any()
+
+
+def test_bbox_identity_and_empty(orc):
+    # Two unit boxes under identity: the group box is their union.
+    l4 = np.array([[0, 0, 0, 1], [2, -1, 5, 1]], F)
+    e4 = np.array([[1, 1, 1, 0], [1, 2, 3, 0]], F)
+    box = orc.bbox_group(l4, e4, [0, 0], ID[None], )
+    assert np.array_equal(box, np.array([0, -1, 0, 3, 1, 8], F))
+    # Empty group: Box4f stays (FLT_MAX, -FLT_MAX) and Box3f(lower, upper) turns it
+    # into (-FLT_MAX, +FLT_MAX) (Boxnt.h:213-219).
+    fmax = np.finfo(F).max
+    empty = orc.bbox_group(np.zeros((0, 4), F), np.zeros((0, 4), F), np.zeros(0, np.uint64), ID[None])
+    assert np.array_equal(empty, np.array([-fmax] * 3 + [fmax] * 3, F))
+
+
+def test_bbox_rotated_box(orc):
+    # 90-degree rotation about z (row vectors: x -> y, y -> -x) then translation (10,0,0):
+    # box [0,1]x[0,2]x[0,3] maps to x in [8,10], y in [0,1], z in [0,3].
+    m = np.array([[0, 1, 0, 0], [-1, 0, 0, 0], [0, 0, 1, 0], [10, 0, 0, 1]], F)
+    box = orc.bbox_group(np.array([[0, 0, 0, 1]], F), np.array([[1, 2, 3, 0]], F), [0], m[None])
+    assert np.array_equal(box, np.array([8, 0, 0, 10, 1, 3], F))
+
+
+def test_bbox_signed_zero_first_wins(orc):
+    # Box4f::update replaces only on a strict compare, so among equal values the
+    # first one seen stays.  Object A (identity, lower 0) has corner x = +0.
+    # Object B uses M = identity with M[3][0] = -0 and lower (-0,-1,-1): every term
+    # of point.x is -0, so its first corner has x = -0 (later corners give +0).
+    mb = ID.copy()
+    mb[3, 0] = -0.0
+    l4 = np.array([[0, 0, 0, 1], [-0.0, -1, -1, 1]], F)
+    e4 = np.array([[1, 1, 1, 0], [1, 1, 1, 0]], F)
+    mats = np.stack([ID, mb])
+    ab = orc.bbox_group(l4, e4, [0, 1], mats)
+    assert ab[0] == 0 and not np.signbit(ab[0])
+    ba = orc.bbox_group(l4[::-1].copy(), e4, [1, 0], mats)
+    assert ba[0] == 0 and np.signbit(ba[0])
