@@ -143,12 +143,13 @@ def main():
     h_vps_p = h_vps.ctypes.data_as(C.POINTER(C.c_float))
 
     def issue_batch():
-        _lib.check(lib.dpc_cull_batch(plan, scene.box_a.ptr, scene.box_b.ptr, None, scene.planes.ptr, scene.ld,
-                                      d_vps.ptr, masks.ptr, stream.handle))
+        _lib.check(lib.dpc_cull_batch(plan, scene.box_a.ptr, scene.box_b.ptr, None, scene.n_objects,
+                                      scene.planes.ptr, scene.ld, d_vps.ptr, masks.ptr, bs.mask_words, stream.handle))
 
     def issue_per_view():
-        _lib.check(lib.dpc_cull_per_view(plan, scene.box_a.ptr, scene.box_b.ptr, None, scene.planes.ptr, scene.ld,
-                                         h_vps_p, masks_pv.ptr, stream.handle))
+        _lib.check(lib.dpc_cull_per_view(plan, scene.box_a.ptr, scene.box_b.ptr, None, scene.n_objects,
+                                         scene.planes.ptr, scene.ld, h_vps_p, masks_pv.ptr, bs.mask_words,
+                                         stream.handle))
 
     graph = dev.Graph(stream, issue_batch)
     wall_b, k_mean_b, k_med_b = run_timed(dev, lib, stream, args.steps, args.warmup, lambda: graph.launch(stream))

@@ -149,13 +149,16 @@ int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const 
 int dpc_batch_plan_destroy(dpc_batch_plan* plan);
 /* Total mask words the pairs address (max offset + words), tests = sum of |group| over pairs. */
 int dpc_batch_plan_info(const dpc_batch_plan* plan, uint64_t* mask_words, uint64_t* tests, uint64_t* work_items);
-/* d_vps: n_frusta x 16 floats in device memory (update it in place between graph replays). */
+/* n_objects: length of the box (and index) arrays; every group must lie inside it.
+ * d_vps: n_frusta x 16 floats in device memory (update it in place between graph replays).
+ * d_masks holds mask_words 64-bit words, at least the plan's mask_words. */
 int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
-                   const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks, void* stream);
-/* The per-view form of the same work: one launch per frustum over its groups. */
+                   size_t n_objects, const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks,
+                   size_t mask_words, void* stream);
+/* The per-view form of the same work: one launch per frustum over its groups (h_vps in host memory). */
 int dpc_cull_per_view(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b,
-                      const uint32_t* d_xform, const float* d_planes, size_t ld, const float* h_vps,
-                      uint64_t* d_masks, void* stream);
+                      const uint32_t* d_xform, size_t n_objects, const float* d_planes, size_t ld,
+                      const float* h_vps, uint64_t* d_masks, size_t mask_words, void* stream);
 
 /* ---- transform tree propagation ----------------------------------------------
  * dp::transform::Tree::compute (dp/transform/src/Tree.cpp:133-166) on a tree
@@ -173,10 +176,11 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
                        const uint64_t* h_level_offsets, size_t n_levels, const uint8_t* d_dirty_local,
                        uint8_t* d_dirty_world, void* stream);
 /* Packs one byte per slot into BitArray words through a slot->index map:
- * bit d_index_of_slot[s] of d_bits = d_bytes[s] != 0 (d_bits must be cleared first).
+ * bit d_index_of_slot[s] of d_bits = d_bytes[s] != 0 (d_bits must be cleared first;
+ * it holds n_bits bits, and indices >= n_bits are skipped).
  * Produces the EventWorldMatricesChanged bitset (dp/transform/Tree.h:48-60). */
 int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of_slot, size_t n_slots,
-                              uint32_t* d_bits, void* stream);
+                              uint32_t* d_bits, size_t n_bits, void* stream);
 
 /* ---- result diff (ResultBitSet::updateChanged, dp/culling/src/ResultBitSet.cpp:61-108) ---
  * changed = old ^ new over n bits; the indices of changed bits are written to

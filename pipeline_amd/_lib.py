@@ -75,10 +75,10 @@ _SIGNATURES = {
                                   C.POINTER(BatchConfig), C.POINTER(P)]),
     "dpc_batch_plan_destroy": (I, [P]),
     "dpc_batch_plan_info": (I, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
-    "dpc_cull_batch": (I, [P, P, P, P, P, SZ, P, P, P]),
-    "dpc_cull_per_view": (I, [P, P, P, P, P, SZ, FP, P, P]),
+    "dpc_cull_batch": (I, [P, P, P, P, SZ, P, SZ, P, P, SZ, P]),
+    "dpc_cull_per_view": (I, [P, P, P, P, SZ, P, SZ, FP, P, SZ, P]),
     "dpc_tree_propagate": (I, [P, P, SZ, P, C.POINTER(U64), SZ, P, P, P]),
-    "dpc_bytes_to_bits_indexed": (I, [P, P, SZ, P, P]),
+    "dpc_bytes_to_bits_indexed": (I, [P, P, SZ, P, SZ, P]),
     "dpc_result_diff_workspace": (SZ, [SZ]),
     "dpc_result_diff": (I, [P, P, SZ, P, P, I, P, P]),
     "dpc_bbox_workspace": (SZ, [SZ]),

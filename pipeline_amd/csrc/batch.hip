@@ -139,6 +139,7 @@ struct dpc_batch_plan {
   uint32_t n_frusta = 0;
   uint64_t mask_words = 0;
   uint64_t tests = 0;
+  uint64_t object_end = 0;  // one past the last object any pair reads
   uint32_t n_items = 0;
   dpc::WorkItem* d_items = nullptr;
   dpc::KPair* d_pairs = nullptr;
@@ -255,6 +256,7 @@ int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const 
     const dpc_group_range& g = groups[p.group];
     const uint64_t words = (g.count + 63) / 64;
     plan->mask_words = std::max(plan->mask_words, p.mask_word_offset + words);
+    plan->object_end = std::max(plan->object_end, g.first_object + g.count);
     plan->tests += g.count;
     for (uint64_t w = 0; w < words; ++w) {
       ViewTile vt{};
@@ -310,11 +312,34 @@ int dpc_batch_plan_info(const dpc_batch_plan* plan, uint64_t* mask_words, uint64
   return DPC_OK;
 }
 
+}  // extern "C"
+
+namespace {
+int check_extents(const dpc_batch_plan* plan, size_t n_objects, const uint32_t* d_xform, size_t ld, size_t mask_words,
+                  const char* who)
+{
+  if (plan->object_end > n_objects)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: groups reach object %llu but the arrays hold %zu", who,
+                     static_cast<unsigned long long>(plan->object_end), n_objects);
+  if (!d_xform && ld < plan->object_end)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: objects without indices need ld >= %llu", who,
+                     static_cast<unsigned long long>(plan->object_end));
+  if (mask_words < plan->mask_words)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: mask buffer holds %zu words, the pairs need %llu", who,
+                     mask_words, static_cast<unsigned long long>(plan->mask_words));
+  return DPC_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
-                   const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks, void* stream)
+                   size_t n_objects, const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks,
+                   size_t mask_words, void* stream)
 {
   if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_batch: null plan");
   if (plan->n_items == 0) return DPC_OK;
+  if (int rc = check_extents(plan, n_objects, d_xform, ld, mask_words, "dpc_cull_batch")) return rc;
   if (!d_box_a || !d_box_b || !d_planes || !d_vps || !d_masks)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_batch: null pointer argument");
   if (!aligned(d_box_a, 16) || !aligned(d_box_b, 8) || !aligned(d_planes, 16) || !aligned(d_masks, 8) ||
@@ -350,9 +375,11 @@ int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float
 }
 
 int dpc_cull_per_view(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
-                      const float* d_planes, size_t ld, const float* h_vps, uint64_t* d_masks, void* stream)
+                      size_t n_objects, const float* d_planes, size_t ld, const float* h_vps, uint64_t* d_masks,
+                      size_t mask_words, void* stream)
 {
   if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_per_view: null plan");
+  if (int rc = check_extents(plan, n_objects, d_xform, ld, mask_words, "dpc_cull_per_view")) return rc;
   if (!d_box_a || !d_box_b || !d_planes || !h_vps || !d_masks)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_per_view: null pointer argument");
   if (!aligned(d_box_a, 16) || !aligned(d_box_b, 8) || !aligned(d_planes, 16) || !aligned(d_masks, 8))

@@ -74,11 +74,12 @@ __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const 
 }
 
 __global__ void bytes_to_bits_kernel(const uint8_t* __restrict__ bytes, const uint32_t* __restrict__ index_of_slot,
-                                     size_t n, uint32_t* __restrict__ bits)
+                                     size_t n, uint32_t* __restrict__ bits, size_t n_bits)
 {
   const size_t s = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n || bytes[s] == 0) return;
-  const uint32_t i = index_of_slot ? index_of_slot[s] : static_cast<uint32_t>(s);
+  const size_t i = index_of_slot ? static_cast<size_t>(index_of_slot[s]) : s;
+  if (i >= n_bits) return;
   atomicOr(&bits[i >> 5], 1u << (i & 31));
 }
 
@@ -136,12 +137,12 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
 }
 
 int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of_slot, size_t n_slots,
-                              uint32_t* d_bits, void* stream)
+                              uint32_t* d_bits, size_t n_bits, void* stream)
 {
-  if (n_slots == 0) return DPC_OK;
+  if (n_slots == 0 || n_bits == 0) return DPC_OK;
   if (!d_bytes || !d_bits) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bytes_to_bits_indexed: null pointer");
   hipLaunchKernelGGL(bytes_to_bits_kernel, dim3(static_cast<unsigned>((n_slots + 255) / 256)), dim3(256), 0,
-                     as_stream(stream), d_bytes, d_index_of_slot, n_slots, d_bits);
+                     as_stream(stream), d_bytes, d_index_of_slot, n_slots, d_bits, n_bits);
   return check_launch("bytes_to_bits_kernel");
 }
 

@@ -49,14 +49,15 @@ class Batch:
 
     def run_batch(self, stream=None):
         s = self.scene
-        _lib.check(_lib.load().dpc_cull_batch(self.plan, s.box_a.ptr, s.box_b.ptr, None, s.planes.ptr, s.ld,
-                                              self.vps.ptr, self.masks.ptr, stream))
+        _lib.check(_lib.load().dpc_cull_batch(self.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr,
+                                              s.ld, self.vps.ptr, self.masks.ptr, self.masks.shape[0], stream))
 
     def run_per_view(self, stream=None):
         s = self.scene
         h = np.ascontiguousarray(self.bs.vps.reshape(-1), F)
-        _lib.check(_lib.load().dpc_cull_per_view(self.plan, s.box_a.ptr, s.box_b.ptr, None, s.planes.ptr, s.ld,
-                                                 h.ctypes.data_as(C.POINTER(C.c_float)), self.masks.ptr, stream))
+        _lib.check(_lib.load().dpc_cull_per_view(self.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects,
+                                                 s.planes.ptr, s.ld, h.ctypes.data_as(C.POINTER(C.c_float)),
+                                                 self.masks.ptr, self.masks.shape[0], stream))
 
     def result(self):
         return self.masks.download()
@@ -142,6 +143,22 @@ def test_graph_replay_with_moving_frusta(orc):
         assert mismatches(replay, oracle_pairs(orc, bs), bs.mask_words * 64) == 0, f"replay {rep}"
     graph.destroy()
     stream.destroy()
+    b.free()
+
+
+def test_launch_rejects_short_buffers():
+    """Group ranges beyond the object arrays and short mask buffers are errors
+    reported before any launch."""
+    bs = small_scene(10)
+    b = Batch(bs)
+    s = b.scene
+    lib = _lib.load()
+    rc = lib.dpc_cull_batch(b.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects - 1, s.planes.ptr, s.ld,
+                            b.vps.ptr, b.masks.ptr, b.masks.shape[0], None)
+    assert rc == 1 and b"groups reach" in lib.dpc_last_error()
+    rc = lib.dpc_cull_batch(b.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr, s.ld,
+                            b.vps.ptr, b.masks.ptr, b.masks.shape[0] - 1, None)
+    assert rc == 1 and b"mask buffer" in lib.dpc_last_error()
     b.free()
 
 

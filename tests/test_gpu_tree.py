@@ -161,7 +161,7 @@ def test_dirty_propagation(orc):
     bits = dev.DeviceArray(((ts.n_slots + 63) // 64) * 2, np.uint32)
     bits.fill_u32(0)
     d_map = dev.DeviceArray.from_host(ts.index_of_slot)
-    _lib.check(lib.dpc_bytes_to_bits_indexed(t.dirty_world.ptr, d_map.ptr, ts.n_slots, bits.ptr, None))
+    _lib.check(lib.dpc_bytes_to_bits_indexed(t.dirty_world.ptr, d_map.ptr, ts.n_slots, bits.ptr, ts.n_slots, None))
     got_bits = np.unpackbits(bits.download().view(np.uint8), bitorder="little")[: ts.n_slots]
     assert np.array_equal(got_bits, event)
     for d in (d_idx, d_new, bits, d_map):
