@@ -69,39 +69,27 @@ def log(msg):
     sys.stderr.flush()
 
 
-class Timer:
-    """Per-launch HIP event pairs on the launch stream, plus host wall time."""
-
-    def __init__(self, dev, stream, n):
-        self.pairs = [(dev.Event(), dev.Event()) for _ in range(n)]
-        self.stream = stream
-
-    def kernel_ms(self):
-        return [a.elapsed_ms(b) for a, b in self.pairs]
-
-    def destroy(self):
-        for a, b in self.pairs:
-            a.destroy()
-            b.destroy()
-
-
 def run_timed(dev, lib, stream, steps, warmup, issue):
+    """Times `steps` back-to-back issues: host wall clock with the device synchronised
+    on both sides, and one HIP event pair on the launch stream around all of them
+    (device time per step, gaps between launches included)."""
     for _ in range(warmup):
         issue()
     stream.sync()
-    t = Timer(dev, stream, steps)
+    e0, e1 = dev.Event(), dev.Event()
     lib.dpc_device_sync()
     t0 = time.perf_counter()
-    for a, b in t.pairs:
-        a.record(stream.handle)
+    e0.record(stream.handle)
+    for _ in range(steps):
         issue()
-        b.record(stream.handle)
+    e1.record(stream.handle)
     stream.sync()
     lib.dpc_device_sync()
     wall = time.perf_counter() - t0
-    ks = t.kernel_ms()
-    t.destroy()
-    return wall, float(np.mean(ks)), float(np.median(ks))
+    dev_ms = e0.elapsed_ms(e1) / steps
+    e0.destroy()
+    e1.destroy()
+    return wall, dev_ms, dev_ms
 
 
 def main():
@@ -127,32 +115,35 @@ def main():
         f"{len(bs.pair_group)} pairs, {bs.tests} tests ({time.perf_counter() - t_gen:.1f}s)")
     scene = dev.upload_scene(bs.lower, bs.upper, bs.matrices)
     G, P = len(bs.group_first), len(bs.pair_group)
-    groups = (_lib.GroupRange * G)(*[_lib.GroupRange(int(a), int(n)) for a, n in zip(bs.group_first, bs.group_count)])
-    pairs = (_lib.CullPair * P)(*[_lib.CullPair(int(f), int(g), int(o)) for f, g, o in
-                                  zip(bs.pair_frustum, bs.pair_group, bs.pair_offset)])
+    from pipeline_amd import autotune
     cfg = None
     if args.tune:
         cfg = _lib.BatchConfig(*[int(x) for x in args.tune.split(",")])
-    plan = C.c_void_p()
-    _lib.check(lib.dpc_batch_plan_create(groups, G, pairs, P, bs.vps.shape[0], C.byref(cfg) if cfg else None,
-                                         C.byref(plan)))
+    else:
+        cfg = autotune.best_batch_config(bs.n, bs.tests, bs.vps.shape[0])
+    cull_cfg = autotune.best_cull_config(1 << 20)
+    if cull_cfg is not None:
+        _lib.check(lib.dpc_cull_set_config(C.byref(cull_cfg)))
+    tuned = {"batch": list(cfg.__getattribute__(f) for f, _ in cfg._fields_) if cfg else "library default",
+             "cull_waves_per_cu": cull_cfg.waves_per_cu if cull_cfg else "library default"}
+    log(f"launch shapes: {tuned}")
+    plan = dev.batched_plan(bs, cfg)
+    plan.bind([scene])
     d_vps = dev.DeviceArray.from_host(bs.vps.reshape(-1, 16))
     masks = dev.DeviceArray(bs.mask_words, np.uint64)
     masks_pv = dev.DeviceArray(bs.mask_words, np.uint64)
-    h_vps = np.ascontiguousarray(bs.vps.reshape(-1), np.float32)
-    h_vps_p = h_vps.ctypes.data_as(C.POINTER(C.c_float))
 
     def issue_batch():
-        _lib.check(lib.dpc_cull_batch(plan, scene.box_a.ptr, scene.box_b.ptr, None, scene.n_objects,
-                                      scene.planes.ptr, scene.ld, d_vps.ptr, masks.ptr, bs.mask_words, stream.handle))
+        plan.cull(d_vps, masks, stream.handle)
 
     def issue_per_view():
-        _lib.check(lib.dpc_cull_per_view(plan, scene.box_a.ptr, scene.box_b.ptr, None, scene.n_objects,
-                                         scene.planes.ptr, scene.ld, h_vps_p, masks_pv.ptr, bs.mask_words,
-                                         stream.handle))
+        plan.cull_per_view(bs.vps, masks_pv, stream.handle)
 
     graph = dev.Graph(stream, issue_batch)
-    wall_b, k_mean_b, k_med_b = run_timed(dev, lib, stream, args.steps, args.warmup, lambda: graph.launch(stream))
+    # the step: one replay of the captured batched launch
+    wall_b, _, _ = run_timed(dev, lib, stream, args.steps, args.warmup, lambda: graph.launch(stream))
+    # the kernel: back-to-back direct launches between one event pair (roofline time base)
+    _, k_mean_b, k_med_b = run_timed(dev, lib, stream, args.steps, args.warmup, issue_batch)
     wall_v, k_mean_v, k_med_v = run_timed(dev, lib, stream, args.steps, args.warmup, issue_per_view)
     batched = masks.download(stream.handle)
     per_view = masks_pv.download(stream.handle)
@@ -192,29 +183,34 @@ def main():
 
     # ---------------- CPU baseline (oracle, one core, bounded sample) ----------------
     t_cpu0 = time.perf_counter()
-    cpu_tests, cpu_time, pairs_done = 0, 0.0, 0
-    obb_cache = {}
+    cpu_tests, cpu_time, pairs_done, passes = 0, 0.0, 0, 0
     order = rng.permutation(P)
-    for p in order:
-        f, g = int(bs.pair_frustum[p]), int(bs.pair_group[p])
-        a, n = int(bs.group_first[g]), int(bs.group_count[g])
-        t0 = time.perf_counter()
-        if g not in obb_cache:   # GroupCPU::updateOBBs runs once per group while nothing is dirty
-            obb_cache[g] = orc.obb_build_group(l4[a:a + n], e4[a:a + n], np.arange(a, a + n), bs.matrices)
-        orc.cull_obbs(obb_cache[g], bs.vps[f])
-        cpu_time += time.perf_counter() - t0
-        cpu_tests += n
-        pairs_done += 1
-        if cpu_time >= args.cpu_seconds:
-            break
-    obb_cache.clear()
+    while cpu_time < args.cpu_seconds:
+        # One pass = the reference's per-view loop over every (frustum, group) pair:
+        # GroupCPU::updateOBBs once per group (then cached while nothing is dirty),
+        # then the per-object test loop of cull().
+        obb_cache = {}
+        for p in order:
+            f, g = int(bs.pair_frustum[p]), int(bs.pair_group[p])
+            a, n = int(bs.group_first[g]), int(bs.group_count[g])
+            t0 = time.perf_counter()
+            if g not in obb_cache:
+                obb_cache[g] = orc.obb_build_group(l4[a:a + n], e4[a:a + n], np.arange(a, a + n), bs.matrices)
+            orc.cull_obbs(obb_cache[g], bs.vps[f])
+            cpu_time += time.perf_counter() - t0
+            cpu_tests += n
+            pairs_done += 1
+            if cpu_time >= args.cpu_seconds:
+                break
+        obb_cache.clear()
+        passes += 1
     cpu_value = cpu_tests / cpu_time
     log(f"cpu oracle: {cpu_tests} tests in {cpu_time:.1f}s = {cpu_value / 1e6:.2f} M tests/s "
         f"({pairs_done} pairs, {time.perf_counter() - t_cpu0:.1f}s)")
     scene.free()
     for d in (d_vps, masks, masks_pv):
         d.free()
-    lib.dpc_batch_plan_destroy(plan)
+    plan.destroy()
 
     # ---------------- flat single frustum (HBM roofline) ----------------
     if not args.skip_extra:
@@ -312,8 +308,9 @@ def main():
                      "basis": f"{OPS_PER_OBJECT} ops/object + {OPS_PER_TEST} ops/test (unfused FP32 add/mul and "
                               "compares); peak = 256 CU x 64 lanes x 2.4 GHz, no FMA, no packed FP32"},
         "cpu_baseline": {"value": cpu_value, "unit": "tests/s", "cores": 1, "kind": "port",
-                         "sample": f"{cpu_tests} tests: {pairs_done} (frustum, group) pairs of this workload, "
-                                   "OBBs built once per group then the per-object test loop"},
+                         "sample": f"{cpu_tests} tests: {pairs_done} (frustum, group) pairs over {passes} pass(es) of this "
+                                   "workload, OBBs built once per group and pass, then the per-object test loop"},
+        "launch_shapes": tuned,
         "device": info,
         "host": {"cpu": cpu_model(), "nproc": os.cpu_count(),
                  "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None},

@@ -102,6 +102,10 @@ int dpc_matrices_to_aos(const float* d_planes, size_t ld, const uint32_t* d_slot
  * (dp/culling/Manager.h:125). */
 int dpc_matrices_scatter(float* d_planes, size_t ld, const uint32_t* d_idx, const float* d_src, size_t count,
                          void* stream);
+/* d_out[k] = d_table[d_idx[k]], or 0xFFFFFFFF when d_idx[k] >= table_len.  Maps
+ * transform indices to tree slots for groups bound to a GPU tree. */
+int dpc_gather_u32(const uint32_t* d_table, size_t table_len, const uint32_t* d_idx, size_t n, uint32_t* d_out,
+                   void* stream);
 /* ManagerBitSet::objectSetBoundingBox (dp/culling/src/ManagerBitSet.cpp:99-109)
  * for n boxes at once: d_lower_upper is n x 6 floats (lower.xyz, upper.xyz);
  * extent = upper - lower in FP32. */
@@ -117,13 +121,36 @@ int dpc_boxes_from_lower_upper(const float* d_lower_upper, size_t n, float* d_bo
 int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
              const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, void* stream);
 
+/* Launch shape of dpc_cull (process-wide; chosen by the autotuner).
+ * waves_per_cu = 0: one wave per 64-object tile over a full grid;
+ * waves_per_cu > 0: a persistent grid of that many waves per CU, each wave
+ * streaming over tiles with the next tile's loads in flight. */
+typedef struct dpc_cull_config {
+  uint32_t waves_per_cu;
+} dpc_cull_config;
+int dpc_cull_get_config(dpc_cull_config* cfg);
+int dpc_cull_set_config(const dpc_cull_config* cfg);
+
 /* ---- batched multi-frustum culling -------------------------------------------
  * Many cull(group, result, viewProjection) calls (dp/culling/Manager.h:139) in
- * one persistent launch.  A group is a contiguous object range; a pair asks
- * for (frustum, group) and names where its mask starts in d_masks (in 64-bit
+ * one persistent launch.  Objects live in "scenes" (device SoA arrays as above);
+ * a group is a contiguous object range of one scene; a pair asks for
+ * (frustum, group) and names where its mask starts in d_masks (in 64-bit
  * words).  Each object's matrix and box are read once per launch and tested
- * against every frustum paired with its group. */
+ * against every frustum paired with its group.  Groups of different scenes can
+ * share one launch (the serving loop batches requests across resident scenes). */
+typedef struct dpc_scene_desc {
+  const float* box_a;     /* n_objects x float4 */
+  const float* box_b;     /* n_objects x float2 */
+  const uint32_t* xform;  /* n_objects matrix indices, or NULL: object i uses matrix i */
+  const float* planes;    /* 4 row planes of ld matrices */
+  uint64_t ld;
+  uint64_t n_objects;
+} dpc_scene_desc;
+
 typedef struct dpc_group_range {
+  uint32_t scene;
+  uint32_t reserved;
   uint64_t first_object;
   uint64_t count;
 } dpc_group_range;
@@ -149,16 +176,16 @@ int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const 
 int dpc_batch_plan_destroy(dpc_batch_plan* plan);
 /* Total mask words the pairs address (max offset + words), tests = sum of |group| over pairs. */
 int dpc_batch_plan_info(const dpc_batch_plan* plan, uint64_t* mask_words, uint64_t* tests, uint64_t* work_items);
-/* n_objects: length of the box (and index) arrays; every group must lie inside it.
- * d_vps: n_frusta x 16 floats in device memory (update it in place between graph replays).
- * d_masks holds mask_words 64-bit words, at least the plan's mask_words. */
-int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
-                   size_t n_objects, const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks,
-                   size_t mask_words, void* stream);
+/* Binds the scene table (host memory, copied): checks every paired group lies
+ * inside its scene, then uploads the table.  Rebind when arrays move. */
+int dpc_batch_plan_bind(dpc_batch_plan* plan, const dpc_scene_desc* scenes, size_t n_scenes);
+/* d_vps: n_frusta x 16 floats in device memory (update it in place between graph
+ * replays); d_masks holds mask_words 64-bit words, at least the plan's mask_words. */
+int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_vps, uint64_t* d_masks, size_t mask_words,
+                   void* stream);
 /* The per-view form of the same work: one launch per frustum over its groups (h_vps in host memory). */
-int dpc_cull_per_view(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b,
-                      const uint32_t* d_xform, size_t n_objects, const float* d_planes, size_t ld,
-                      const float* h_vps, uint64_t* d_masks, size_t mask_words, void* stream);
+int dpc_cull_per_view(const dpc_batch_plan* plan, const float* h_vps, uint64_t* d_masks, size_t mask_words,
+                      void* stream);
 
 /* ---- transform tree propagation ----------------------------------------------
  * dp::transform::Tree::compute (dp/transform/src/Tree.cpp:133-166) on a tree

@@ -83,6 +83,11 @@ def build(verbose: bool = False) -> Path:
     return out
 
 
+def build_cpp_tests() -> None:
+    """The C++ tests of the host mirrors (tests/cpp), linked against the library."""
+    _run(["make", "-s", "-C", str(ROOT / "tests" / "cpp"), "-j4"])
+
+
 def build_oracle() -> None:
     """Compile the CPU oracle (test infrastructure) and, where the reference
     sources are present, its compiled-reference companion oracle/_ref."""

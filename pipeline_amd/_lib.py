@@ -19,12 +19,22 @@ class DpcError(RuntimeError):
         self.code = code
 
 
+class SceneDesc(C.Structure):
+    _fields_ = [("box_a", C.c_void_p), ("box_b", C.c_void_p), ("xform", C.c_void_p), ("planes", C.c_void_p),
+                ("ld", C.c_uint64), ("n_objects", C.c_uint64)]
+
+
 class GroupRange(C.Structure):
-    _fields_ = [("first_object", C.c_uint64), ("count", C.c_uint64)]
+    _fields_ = [("scene", C.c_uint32), ("reserved", C.c_uint32), ("first_object", C.c_uint64),
+                ("count", C.c_uint64)]
 
 
 class CullPair(C.Structure):
     _fields_ = [("frustum", C.c_uint32), ("group", C.c_uint32), ("mask_word_offset", C.c_uint64)]
+
+
+class CullConfig(C.Structure):
+    _fields_ = [("waves_per_cu", C.c_uint32)]
 
 
 class BatchConfig(C.Structure):
@@ -70,13 +80,17 @@ _SIGNATURES = {
     "dpc_matrices_to_aos": (I, [P, SZ, P, SZ, P, SZ, P]),
     "dpc_matrices_scatter": (I, [P, SZ, P, P, SZ, P]),
     "dpc_boxes_from_lower_upper": (I, [P, SZ, P, P, P]),
+    "dpc_gather_u32": (I, [P, SZ, P, SZ, P, P]),
     "dpc_cull": (I, [P, P, P, SZ, P, SZ, FP, P, P]),
+    "dpc_cull_get_config": (I, [C.POINTER(CullConfig)]),
+    "dpc_cull_set_config": (I, [C.POINTER(CullConfig)]),
     "dpc_batch_plan_create": (I, [C.POINTER(GroupRange), SZ, C.POINTER(CullPair), SZ, U32,
                                   C.POINTER(BatchConfig), C.POINTER(P)]),
     "dpc_batch_plan_destroy": (I, [P]),
     "dpc_batch_plan_info": (I, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64)]),
-    "dpc_cull_batch": (I, [P, P, P, P, SZ, P, SZ, P, P, SZ, P]),
-    "dpc_cull_per_view": (I, [P, P, P, P, SZ, P, SZ, FP, P, SZ, P]),
+    "dpc_batch_plan_bind": (I, [P, C.POINTER(SceneDesc), SZ]),
+    "dpc_cull_batch": (I, [P, P, P, SZ, P]),
+    "dpc_cull_per_view": (I, [P, FP, P, SZ, P]),
     "dpc_tree_propagate": (I, [P, P, SZ, P, C.POINTER(U64), SZ, P, P, P]),
     "dpc_bytes_to_bits_indexed": (I, [P, P, SZ, P, SZ, P]),
     "dpc_result_diff_workspace": (SZ, [SZ]),

@@ -35,7 +35,8 @@ struct WorkItem {
   uint32_t tile_word;   // mask word of the tile's first object, relative to its group
   uint32_t pair_begin;  // slice of the pair table
   uint32_t pair_count;
-  uint32_t pad0, pad1;
+  uint32_t scene;       // entry of the scene table
+  uint32_t pad;
 };
 static_assert(sizeof(WorkItem) == 32, "WorkItem layout");
 
@@ -50,75 +51,109 @@ struct ViewTile {
   uint64_t first_object;
   uint64_t mask_word;
   uint32_t count;
-  uint32_t pad0;
-  uint64_t pad1;
+  uint32_t scene;
+  uint64_t pad;
 };
 static_assert(sizeof(ViewTile) == 32, "ViewTile layout");
 
-__device__ __forceinline__ Obb load_obb(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
-                                        const uint32_t* __restrict__ xform, const float4* __restrict__ planes,
-                                        size_t ld, size_t i)
+// Device copy of a dpc_scene_desc.
+struct SceneDev {
+  const float4* box_a;
+  const float2* box_b;
+  const uint32_t* xform;
+  const float4* planes;
+  uint64_t ld;
+  uint64_t n_objects;
+};
+static_assert(sizeof(SceneDev) == 48, "SceneDev layout");
+
+__device__ __forceinline__ size_t matrix_index(const uint32_t* __restrict__ xform, size_t i)
 {
-  const size_t m = xform ? static_cast<size_t>(xform[i]) : i;
-  if (m >= ld) {
-    // A matrix index outside the array is never dereferenced.  A NaN box makes
-    // every clip coordinate NaN, so no cull flag is set and the object stays visible.
-    const float q = __builtin_nanf("");
-    Obb o;
-    o.p = make_float4(q, q, q, q);
-    o.ex = o.p;
-    o.ey = o.p;
-    o.ez = o.p;
-    return o;
+  return xform ? static_cast<size_t>(xform[i]) : i;
+}
+
+__device__ __forceinline__ void load_vp(const float* __restrict__ vps, uint32_t frustum, Mat16& vp)
+{
+#pragma unroll
+  for (int c = 0; c < 16; ++c) vp.m[c] = vps[16 * static_cast<size_t>(frustum) + c];
+}
+
+// The wave-uniform loop state is software-pipelined twice: the objects of the
+// next work item are loaded (vector memory) while the current item is tested,
+// and the next frustum matrix is loaded (scalar memory) while the current one is
+// in use.  Neither wait lands in front of the FP32 work.
+template <int K>
+__device__ __forceinline__ void load_item(const SceneDev* __restrict__ scenes, const WorkItem& item, uint32_t lane,
+                                          RawObj (&raw)[K])
+{
+  const SceneDev sc = scenes[item.scene];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const uint32_t j = lane + kWave * k;
+    const size_t i = item.first_object + j;
+    raw[k] = load_raw(sc.box_a, sc.box_b, sc.planes, sc.ld, i, j < item.count ? matrix_index(sc.xform, i) : ~size_t(0));
   }
-  return make_obb(box_a[i], box_b[i], planes[m], planes[ld + m], planes[2 * ld + m], planes[3 * ld + m]);
 }
 
 template <int K>
 __global__ __launch_bounds__(256) void cull_batch_kernel(const WorkItem* __restrict__ items, uint32_t n_items,
                                                          const KPair* __restrict__ pairs,
-                                                         const float4* __restrict__ box_a,
-                                                         const float2* __restrict__ box_b,
-                                                         const uint32_t* __restrict__ xform,
-                                                         const float4* __restrict__ planes, size_t ld,
+                                                         const SceneDev* __restrict__ scenes,
                                                          const float* __restrict__ vps,
                                                          unsigned long long* __restrict__ masks)
 {
   const uint32_t lane = threadIdx.x & (kWave - 1);
   const uint32_t waves_per_block = blockDim.x / kWave;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + threadIdx.x / kWave);
   const uint32_t n_waves = gridDim.x * waves_per_block;
-  for (uint32_t it = wave; it < n_items; it += n_waves) {
-    const WorkItem item = items[it];
+  uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + threadIdx.x / kWave);
+  if (it >= n_items) return;
+  WorkItem item = items[it];
+  RawObj raw[K];
+  load_item<K>(scenes, item, lane, raw);
+  while (it < n_items) {
+    const uint32_t nit = it + n_waves;
+    WorkItem next_item = item;
+    RawObj next_raw[K];
+    if (nit < n_items) {
+      next_item = items[nit];
+      load_item<K>(scenes, next_item, lane, next_raw);
+    }
     Obb o[K];
     bool valid[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const uint32_t j = lane + kWave * k;
-      valid[k] = j < item.count;
-      if (valid[k]) o[k] = load_obb(box_a, box_b, xform, planes, ld, item.first_object + j);
+      valid[k] = lane + kWave * k < item.count;
+      o[k] = obb_of(raw[k]);
     }
+    KPair pr = pairs[item.pair_begin];
+    Mat16 vp;
+    load_vp(vps, pr.frustum, vp);
     for (uint32_t p = 0; p < item.pair_count; ++p) {
-      const KPair pr = pairs[item.pair_begin + p];
-      Mat16 vp;
-#pragma unroll
-      for (int c = 0; c < 16; ++c) vp.m[c] = vps[16 * static_cast<size_t>(pr.frustum) + c];
+      KPair next_pr = pr;
+      Mat16 next_vp;
+      if (p + 1 < item.pair_count) {
+        next_pr = pairs[item.pair_begin + p + 1];
+        load_vp(vps, next_pr.frustum, next_vp);
+      }
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const bool vis = valid[k] && obb_visible(o[k], vp.m);
+        const bool vis = valid[k] & obb_visible(o[k], vp.m);
         const unsigned long long bits = __ballot(vis);
         if (lane == 0 && static_cast<uint32_t>(kWave * k) < item.count) masks[pr.mask_word + item.tile_word + k] = bits;
       }
+      pr = next_pr;
+      vp = next_vp;
     }
+    it = nit;
+    item = next_item;
+#pragma unroll
+    for (int k = 0; k < K; ++k) raw[k] = next_raw[k];
   }
 }
 
 // Plain per-view launch: one wave per 64-object tile, the frustum by value.
 __global__ __launch_bounds__(256) void cull_tiles_kernel(const ViewTile* __restrict__ tiles, uint32_t n_tiles,
-                                                         const float4* __restrict__ box_a,
-                                                         const float2* __restrict__ box_b,
-                                                         const uint32_t* __restrict__ xform,
-                                                         const float4* __restrict__ planes, size_t ld, Mat16 vp,
+                                                         const SceneDev* __restrict__ scenes, Mat16 vp,
                                                          unsigned long long* __restrict__ masks)
 {
   const uint32_t lane = threadIdx.x & (kWave - 1);
@@ -126,7 +161,11 @@ __global__ __launch_bounds__(256) void cull_tiles_kernel(const ViewTile* __restr
   if (t >= n_tiles) return;
   const ViewTile tile = tiles[t];
   bool vis = false;
-  if (lane < tile.count) vis = obb_visible(load_obb(box_a, box_b, xform, planes, ld, tile.first_object + lane), vp.m);
+  if (lane < tile.count) {
+    const SceneDev sc = scenes[tile.scene];
+    const size_t i = tile.first_object + lane;
+    vis = obb_visible(obb_of(load_raw(sc.box_a, sc.box_b, sc.planes, sc.ld, i, matrix_index(sc.xform, i))), vp.m);
+  }
   const unsigned long long bits = __ballot(vis);
   if (lane == 0) masks[tile.mask_word] = bits;
 }
@@ -139,10 +178,14 @@ struct dpc_batch_plan {
   uint32_t n_frusta = 0;
   uint64_t mask_words = 0;
   uint64_t tests = 0;
-  uint64_t object_end = 0;  // one past the last object any pair reads
   uint32_t n_items = 0;
   dpc::WorkItem* d_items = nullptr;
   dpc::KPair* d_pairs = nullptr;
+  // groups read by at least one pair, for validating bound scenes
+  std::vector<dpc_group_range> used_groups;
+  uint32_t n_scenes_needed = 0;
+  dpc::SceneDev* d_scenes = nullptr;
+  uint32_t n_scenes_bound = 0;
   // per-view form
   std::vector<uint32_t> view_tile_begin;  // n_frusta + 1
   dpc::ViewTile* d_view_tiles = nullptr;
@@ -170,6 +213,7 @@ void free_plan(dpc_batch_plan* p)
   if (p->d_items) (void)hipFree(p->d_items);
   if (p->d_pairs) (void)hipFree(p->d_pairs);
   if (p->d_view_tiles) (void)hipFree(p->d_view_tiles);
+  if (p->d_scenes) (void)hipFree(p->d_scenes);
   delete p;
 }
 
@@ -228,6 +272,7 @@ int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const 
       const uint32_t cnt = static_cast<uint32_t>(std::min<uint64_t>(tile_objects, groups[g].count - t0));
       for (uint64_t q = pb; q < pe; q += chunk) {
         WorkItem w{};
+        w.scene = groups[g].scene;
         w.first_object = groups[g].first_object + t0;
         w.count = cnt;
         w.tile_word = static_cast<uint32_t>(t0 / kWave);
@@ -256,13 +301,13 @@ int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const 
     const dpc_group_range& g = groups[p.group];
     const uint64_t words = (g.count + 63) / 64;
     plan->mask_words = std::max(plan->mask_words, p.mask_word_offset + words);
-    plan->object_end = std::max(plan->object_end, g.first_object + g.count);
     plan->tests += g.count;
     for (uint64_t w = 0; w < words; ++w) {
       ViewTile vt{};
       vt.first_object = g.first_object + 64 * w;
       vt.mask_word = p.mask_word_offset + w;
       vt.count = static_cast<uint32_t>(std::min<uint64_t>(64, g.count - 64 * w));
+      vt.scene = g.scene;
       per_view[p.frustum].push_back(vt);
     }
   }
@@ -273,6 +318,15 @@ int dpc_batch_plan_create(const dpc_group_range* groups, size_t n_groups, const 
     flat.insert(flat.end(), per_view[f].begin(), per_view[f].end());
   }
   plan->view_tile_begin[n_frusta] = static_cast<uint32_t>(flat.size());
+  {
+    std::vector<bool> used(n_groups, false);
+    for (size_t k = 0; k < n_pairs; ++k) used[pairs[k].group] = true;
+    for (size_t g = 0; g < n_groups; ++g) {
+      if (!used[g] || groups[g].count == 0) continue;
+      plan->used_groups.push_back(groups[g]);
+      plan->n_scenes_needed = std::max(plan->n_scenes_needed, groups[g].scene + 1);
+    }
+  }
 
   hipError_t e = hipSuccess;
   if (!items.empty()) {
@@ -315,15 +369,12 @@ int dpc_batch_plan_info(const dpc_batch_plan* plan, uint64_t* mask_words, uint64
 }  // extern "C"
 
 namespace {
-int check_extents(const dpc_batch_plan* plan, size_t n_objects, const uint32_t* d_xform, size_t ld, size_t mask_words,
-                  const char* who)
+int check_bound(const dpc_batch_plan* plan, size_t mask_words, const char* who)
 {
-  if (plan->object_end > n_objects)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: groups reach object %llu but the arrays hold %zu", who,
-                     static_cast<unsigned long long>(plan->object_end), n_objects);
-  if (!d_xform && ld < plan->object_end)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: objects without indices need ld >= %llu", who,
-                     static_cast<unsigned long long>(plan->object_end));
+  if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: null plan", who);
+  if (plan->n_scenes_bound < plan->n_scenes_needed)
+    return set_error(DPC_ERR_STATE, "%s: the plan reads %u scenes but %u are bound (dpc_batch_plan_bind)", who,
+                     plan->n_scenes_needed, plan->n_scenes_bound);
   if (mask_words < plan->mask_words)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: mask buffer holds %zu words, the pairs need %llu", who,
                      mask_words, static_cast<unsigned long long>(plan->mask_words));
@@ -333,18 +384,59 @@ int check_extents(const dpc_batch_plan* plan, size_t n_objects, const uint32_t* 
 
 extern "C" {
 
-int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
-                   size_t n_objects, const float* d_planes, size_t ld, const float* d_vps, uint64_t* d_masks,
-                   size_t mask_words, void* stream)
+int dpc_batch_plan_bind(dpc_batch_plan* plan, const dpc_scene_desc* scenes, size_t n_scenes)
 {
-  if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_batch: null plan");
+  if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_batch_plan_bind: null plan");
+  if (n_scenes && !scenes) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_batch_plan_bind: null scene table");
+  if (n_scenes < plan->n_scenes_needed)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_batch_plan_bind: the plan reads %u scenes, %zu given",
+                     plan->n_scenes_needed, n_scenes);
+  for (size_t k = 0; k < n_scenes; ++k) {
+    const dpc_scene_desc& d = scenes[k];
+    if (d.n_objects && (!d.box_a || !d.box_b || !d.planes || !aligned(d.box_a, 16) || !aligned(d.box_b, 8) ||
+                        !aligned(d.planes, 16)))
+      return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_batch_plan_bind: scene %zu has a null or misaligned array", k);
+  }
+  for (const dpc_group_range& g : plan->used_groups) {
+    const dpc_scene_desc& d = scenes[g.scene];
+    if (g.first_object + g.count > d.n_objects)
+      return set_error(DPC_ERR_INVALID_ARGUMENT,
+                       "dpc_batch_plan_bind: a group reaches object %llu of scene %u, which holds %llu",
+                       static_cast<unsigned long long>(g.first_object + g.count), g.scene,
+                       static_cast<unsigned long long>(d.n_objects));
+    if (!d.xform && d.ld < g.first_object + g.count)
+      return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_batch_plan_bind: scene %u has no indices and ld < objects",
+                       g.scene);
+  }
+  std::vector<SceneDev> table(n_scenes);
+  for (size_t k = 0; k < n_scenes; ++k) {
+    table[k].box_a = reinterpret_cast<const float4*>(scenes[k].box_a);
+    table[k].box_b = reinterpret_cast<const float2*>(scenes[k].box_b);
+    table[k].xform = scenes[k].xform;
+    table[k].planes = reinterpret_cast<const float4*>(scenes[k].planes);
+    table[k].ld = scenes[k].ld;
+    table[k].n_objects = scenes[k].n_objects;
+  }
+  if (n_scenes > plan->n_scenes_bound || !plan->d_scenes) {
+    if (plan->d_scenes) (void)hipFree(plan->d_scenes);
+    plan->d_scenes = nullptr;
+    plan->n_scenes_bound = 0;
+    DPC_HIP_TRY(hipMalloc(&plan->d_scenes, std::max<size_t>(n_scenes, 1) * sizeof(SceneDev)), "hipMalloc scenes");
+  }
+  if (n_scenes)
+    DPC_HIP_TRY(hipMemcpy(plan->d_scenes, table.data(), n_scenes * sizeof(SceneDev), hipMemcpyHostToDevice),
+                "upload scene table");
+  plan->n_scenes_bound = static_cast<uint32_t>(n_scenes);
+  return DPC_OK;
+}
+
+int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_vps, uint64_t* d_masks, size_t mask_words,
+                   void* stream)
+{
+  if (int rc = check_bound(plan, mask_words, "dpc_cull_batch")) return rc;
   if (plan->n_items == 0) return DPC_OK;
-  if (int rc = check_extents(plan, n_objects, d_xform, ld, mask_words, "dpc_cull_batch")) return rc;
-  if (!d_box_a || !d_box_b || !d_planes || !d_vps || !d_masks)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_batch: null pointer argument");
-  if (!aligned(d_box_a, 16) || !aligned(d_box_b, 8) || !aligned(d_planes, 16) || !aligned(d_masks, 8) ||
-      !aligned(d_vps, 4))
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_batch: misaligned pointer");
+  if (!d_vps || !d_masks || !aligned(d_masks, 8) || !aligned(d_vps, 4))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_batch: null or misaligned frusta/masks");
   const dpc_batch_config& c = plan->cfg;
   const uint32_t waves_per_block = c.block_threads / kWave;
   uint64_t blocks = static_cast<uint64_t>(device_cu_count()) * c.waves_per_cu / waves_per_block;
@@ -352,41 +444,31 @@ int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_box_a, const float
   if (blocks > needed) blocks = needed;
   if (blocks == 0) blocks = 1;
   const dim3 grid(static_cast<unsigned>(blocks)), block(c.block_threads);
-  const auto* a = reinterpret_cast<const float4*>(d_box_a);
-  const auto* b = reinterpret_cast<const float2*>(d_box_b);
-  const auto* p = reinterpret_cast<const float4*>(d_planes);
   auto* m = reinterpret_cast<unsigned long long*>(d_masks);
   hipStream_t s = as_stream(stream);
   switch (c.objects_per_lane) {
     case 1:
-      hipLaunchKernelGGL(cull_batch_kernel<1>, grid, block, 0, s, plan->d_items, plan->n_items, plan->d_pairs, a, b,
-                         d_xform, p, ld, d_vps, m);
+      hipLaunchKernelGGL(cull_batch_kernel<1>, grid, block, 0, s, plan->d_items, plan->n_items, plan->d_pairs,
+                         plan->d_scenes, d_vps, m);
       break;
     case 2:
-      hipLaunchKernelGGL(cull_batch_kernel<2>, grid, block, 0, s, plan->d_items, plan->n_items, plan->d_pairs, a, b,
-                         d_xform, p, ld, d_vps, m);
+      hipLaunchKernelGGL(cull_batch_kernel<2>, grid, block, 0, s, plan->d_items, plan->n_items, plan->d_pairs,
+                         plan->d_scenes, d_vps, m);
       break;
     default:
-      hipLaunchKernelGGL(cull_batch_kernel<4>, grid, block, 0, s, plan->d_items, plan->n_items, plan->d_pairs, a, b,
-                         d_xform, p, ld, d_vps, m);
+      hipLaunchKernelGGL(cull_batch_kernel<4>, grid, block, 0, s, plan->d_items, plan->n_items, plan->d_pairs,
+                         plan->d_scenes, d_vps, m);
       break;
   }
   return check_launch("cull_batch_kernel");
 }
 
-int dpc_cull_per_view(const dpc_batch_plan* plan, const float* d_box_a, const float* d_box_b, const uint32_t* d_xform,
-                      size_t n_objects, const float* d_planes, size_t ld, const float* h_vps, uint64_t* d_masks,
-                      size_t mask_words, void* stream)
+int dpc_cull_per_view(const dpc_batch_plan* plan, const float* h_vps, uint64_t* d_masks, size_t mask_words,
+                      void* stream)
 {
-  if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_per_view: null plan");
-  if (int rc = check_extents(plan, n_objects, d_xform, ld, mask_words, "dpc_cull_per_view")) return rc;
-  if (!d_box_a || !d_box_b || !d_planes || !h_vps || !d_masks)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_per_view: null pointer argument");
-  if (!aligned(d_box_a, 16) || !aligned(d_box_b, 8) || !aligned(d_planes, 16) || !aligned(d_masks, 8))
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_per_view: misaligned pointer");
-  const auto* a = reinterpret_cast<const float4*>(d_box_a);
-  const auto* b = reinterpret_cast<const float2*>(d_box_b);
-  const auto* p = reinterpret_cast<const float4*>(d_planes);
+  if (int rc = check_bound(plan, mask_words, "dpc_cull_per_view")) return rc;
+  if (!h_vps || !d_masks || !aligned(d_masks, 8))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_per_view: null or misaligned frusta/masks");
   auto* m = reinterpret_cast<unsigned long long*>(d_masks);
   hipStream_t s = as_stream(stream);
   for (uint32_t f = 0; f < plan->n_frusta; ++f) {
@@ -396,8 +478,8 @@ int dpc_cull_per_view(const dpc_batch_plan* plan, const float* d_box_a, const fl
     std::memcpy(vp.m, h_vps + 16 * static_cast<size_t>(f), sizeof vp.m);
     const uint32_t n = t1 - t0;
     const unsigned blocks = (n + 3) / 4;
-    hipLaunchKernelGGL(cull_tiles_kernel, dim3(blocks), dim3(256), 0, s, plan->d_view_tiles + t0, n, a, b, d_xform, p,
-                       ld, vp, m);
+    hipLaunchKernelGGL(cull_tiles_kernel, dim3(blocks), dim3(256), 0, s, plan->d_view_tiles + t0, n, plan->d_scenes,
+                       vp, m);
     int rc = check_launch("cull_tiles_kernel");
     if (rc) return rc;
   }

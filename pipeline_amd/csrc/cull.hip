@@ -5,6 +5,7 @@
 // (dp/culling/cpu/src/ManagerImpl.cpp:467-518), GroupCPU::updateOBBs (:114-162),
 // ManagerBitSet::calculateBoundingBox (dp/culling/src/ManagerBitSet.cpp:268-306)
 // and ResultBitSet::updateChanged (dp/culling/src/ResultBitSet.cpp:61-108).
+#include <atomic>
 #include <cfloat>
 
 #include "cull_math.hpp"
@@ -17,9 +18,10 @@ namespace {
 
 constexpr unsigned kCullThreads = 256;
 
-// One object per lane; one 64-bit mask word per wave from a ballot.
-// The object's four matrix rows come from four row planes, so every load of
-// the wave is a contiguous 1 KiB (rows) or 1 KiB / 512 B (boxes) segment.
+// Plain form: one object per lane, one 64-object tile per wave, a grid that
+// covers all tiles.  The object's four matrix rows come from four row planes,
+// so every load of the wave is one contiguous 1 KiB (rows, box_a) or 512 B
+// (box_b) segment; the visibility bits leave as one ballot word per wave.
 template <bool Indexed>
 __global__ __launch_bounds__(kCullThreads) void cull_kernel(const float4* __restrict__ box_a,
                                                             const float2* __restrict__ box_b,
@@ -31,15 +33,52 @@ __global__ __launch_bounds__(kCullThreads) void cull_kernel(const float4* __rest
   bool vis = false;
   if (i < n) {
     const size_t m = Indexed ? static_cast<size_t>(xform[i]) : i;
-    if (m < ld) {
-      const Obb o = make_obb(box_a[i], box_b[i], planes[m], planes[ld + m], planes[2 * ld + m], planes[3 * ld + m]);
-      vis = obb_visible(o, vp.m);
-    } else {
-      vis = true;  // a matrix index outside the array is never dereferenced; such objects stay visible
-    }
+    vis = obb_visible(obb_of(load_raw(box_a, box_b, planes, ld, i, m)), vp.m);
   }
   const unsigned long long bits = __ballot(vis);
   if ((threadIdx.x & (kWave - 1)) == 0 && i < n) mask[i >> 6] = bits;
+}
+
+// Streaming form: a persistent grid in which every wave walks tiles t, t+W,
+// t+2W, ... and issues the loads of its next tile before it computes the
+// current one, so memory and the FP32 pipeline stay busy together.  With
+// indices, the index of the tile after next is loaded one step earlier still,
+// so the dependent row gather never waits inside the loop.
+template <bool Indexed>
+__global__ __launch_bounds__(kCullThreads) void cull_stream_kernel(const float4* __restrict__ box_a,
+                                                                   const float2* __restrict__ box_b,
+                                                                   const uint32_t* __restrict__ xform, size_t n,
+                                                                   const float4* __restrict__ planes, size_t ld,
+                                                                   Mat16 vp, unsigned long long* __restrict__ mask)
+{
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  const size_t waves_per_block = blockDim.x / kWave;
+  const size_t W = static_cast<size_t>(gridDim.x) * waves_per_block;
+  const size_t n_tiles = (n + kWave - 1) / kWave;
+  size_t t = static_cast<size_t>(blockIdx.x) * waves_per_block + threadIdx.x / kWave;
+  if (t >= n_tiles) return;
+  // matrix index of this lane's object in a tile (n: no object)
+  auto index_of = [&](size_t tile) -> size_t {
+    const size_t i = tile * kWave + lane;
+    if (tile >= n_tiles || i >= n) return ~size_t(0);
+    return Indexed ? static_cast<size_t>(xform[i]) : i;
+  };
+  size_t m_cur = index_of(t);
+  size_t m_next = index_of(t + W);
+  RawObj cur = load_raw(box_a, box_b, planes, ld, t * kWave + lane, m_cur);
+  while (t < n_tiles) {
+    const size_t tn = t + W;
+    const size_t m_after = Indexed ? index_of(tn + W) : ~size_t(0);
+    RawObj nxt;
+    if (tn < n_tiles) nxt = load_raw(box_a, box_b, planes, ld, tn * kWave + lane, m_next);
+    const size_t i = t * kWave + lane;
+    const bool vis = i < n && obb_visible(obb_of(cur), vp.m);
+    const unsigned long long bits = __ballot(vis);
+    if (lane == 0) mask[t] = bits;
+    cur = nxt;
+    m_next = Indexed ? m_after : index_of(tn + W);
+    t = tn;
+  }
 }
 
 // Row planes <- AoS matrices with a byte stride.  One thread per (matrix, row).
@@ -77,6 +116,15 @@ __global__ void scatter_kernel(float4* __restrict__ planes, size_t ld, const uin
   const size_t k = t >> 2, r = t & 3;
   const size_t s = idx[k];
   if (s < ld) planes[r * ld + s] = src[k * 4 + r];
+}
+
+__global__ void gather_u32_kernel(const uint32_t* __restrict__ table, size_t table_len,
+                                  const uint32_t* __restrict__ idx, size_t n, uint32_t* __restrict__ out)
+{
+  const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t i = idx[k];
+  out[k] = i < table_len ? table[i] : 0xFFFFFFFFu;
 }
 
 __global__ void boxes_kernel(const float* __restrict__ lu, size_t n, float4* __restrict__ box_a,
@@ -321,6 +369,15 @@ unsigned grid_for(size_t work, unsigned per_block)
   return static_cast<unsigned>((work + per_block - 1) / per_block);
 }
 
+std::atomic<uint32_t> g_cull_waves_per_cu{16};
+
+dpc_cull_config current_cull_config()
+{
+  dpc_cull_config c;
+  c.waves_per_cu = g_cull_waves_per_cu.load(std::memory_order_relaxed);
+  return c;
+}
+
 }  // namespace
 }  // namespace dpc
 
@@ -341,18 +398,49 @@ int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
                      n_objects, ld);
   Mat16 m;
   for (int k = 0; k < 16; ++k) m.m[k] = vp[k];
-  const dim3 grid(grid_for(n_objects, kCullThreads));
   const auto* a = reinterpret_cast<const float4*>(d_box_a);
   const auto* b = reinterpret_cast<const float2*>(d_box_b);
   const auto* p = reinterpret_cast<const float4*>(d_planes);
   auto* out = reinterpret_cast<unsigned long long*>(d_mask);
-  if (d_xform)
-    hipLaunchKernelGGL(cull_kernel<true>, grid, dim3(kCullThreads), 0, as_stream(stream), a, b, d_xform, n_objects, p,
-                       ld, m, out);
-  else
-    hipLaunchKernelGGL(cull_kernel<false>, grid, dim3(kCullThreads), 0, as_stream(stream), a, b, d_xform, n_objects, p,
-                       ld, m, out);
+  const dpc_cull_config cfg = current_cull_config();
+  const size_t tiles = (n_objects + kWave - 1) / kWave;
+  if (cfg.waves_per_cu == 0) {
+    const dim3 grid(grid_for(n_objects, kCullThreads));
+    if (d_xform)
+      hipLaunchKernelGGL(cull_kernel<true>, grid, dim3(kCullThreads), 0, as_stream(stream), a, b, d_xform, n_objects,
+                         p, ld, m, out);
+    else
+      hipLaunchKernelGGL(cull_kernel<false>, grid, dim3(kCullThreads), 0, as_stream(stream), a, b, d_xform, n_objects,
+                         p, ld, m, out);
+  } else {
+    const size_t waves_per_block = kCullThreads / kWave;
+    size_t blocks = static_cast<size_t>(device_cu_count()) * cfg.waves_per_cu / waves_per_block;
+    const size_t needed = (tiles + waves_per_block - 1) / waves_per_block;
+    if (blocks > needed) blocks = needed;
+    if (blocks == 0) blocks = 1;
+    if (d_xform)
+      hipLaunchKernelGGL(cull_stream_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kCullThreads), 0,
+                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
+    else
+      hipLaunchKernelGGL(cull_stream_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(kCullThreads), 0,
+                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
+  }
   return check_launch("cull_kernel");
+}
+
+int dpc_cull_get_config(dpc_cull_config* cfg)
+{
+  if (!cfg) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_get_config: null output");
+  *cfg = current_cull_config();
+  return DPC_OK;
+}
+
+int dpc_cull_set_config(const dpc_cull_config* cfg)
+{
+  if (!cfg || cfg->waves_per_cu > 64)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull_set_config: waves_per_cu must be 0..64");
+  g_cull_waves_per_cu.store(cfg->waves_per_cu, std::memory_order_relaxed);
+  return DPC_OK;
 }
 
 int dpc_matrices_from_aos(const void* d_aos, size_t stride_bytes, size_t count, float* d_planes, size_t ld,
@@ -391,6 +479,17 @@ int dpc_matrices_scatter(float* d_planes, size_t ld, const uint32_t* d_idx, cons
   hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(count * 4, 256)), dim3(256), 0, as_stream(stream),
                      reinterpret_cast<float4*>(d_planes), ld, d_idx, reinterpret_cast<const float4*>(d_src), count);
   return check_launch("scatter_kernel");
+}
+
+int dpc_gather_u32(const uint32_t* d_table, size_t table_len, const uint32_t* d_idx, size_t n, uint32_t* d_out,
+                   void* stream)
+{
+  if (n == 0) return DPC_OK;
+  if (!d_idx || !d_out || (table_len && !d_table))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_gather_u32: null pointer");
+  hipLaunchKernelGGL(gather_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream), d_table, table_len,
+                     d_idx, n, d_out);
+  return check_launch("gather_u32_kernel");
 }
 
 int dpc_boxes_from_lower_upper(const float* d_lower_upper, size_t n, float* d_box_a, float* d_box_b, void* stream)

@@ -114,27 +114,30 @@ __device__ __forceinline__ Obb make_obb(float4 box_a, float2 box_b, float4 r0, f
   return o;
 }
 
-// determineCullFlags (ManagerImpl.cpp:199-229) folded into the AND over corners:
-// a[k] accumulates "every corner so far carries flag bit k".  The else-if of the
-// reference is kept: the "high" flag of an axis is set only when the "low" one is not.
-struct PlaneAnd {
-  bool a0, a1, a2, a3, a4, a5;
+// determineCullFlags (ManagerImpl.cpp:199-229) folded into the AND over the
+// eight corners.  Per axis the reference sets the "low" flag when c <= -w and,
+// only otherwise, the "high" flag when w <= c.  Over the corners:
+//   low bit common to all corners   = AND_k (c_k <= -w_k)
+//   high bit common to all corners  = AND_k (!(c_k <= -w_k) && w_k <= c_k)
+//                                   = !(OR_k (c_k <= -w_k)) && AND_k (w_k <= c_k)
+// so two compares per axis and corner suffice (48 per test), with the AND/OR
+// accumulation carried in wave-wide lane masks.
+struct PlaneAcc {
+  bool lo_all[3], lo_any[3], hi_all[3];
 };
 
-__device__ __forceinline__ void corner_flags(float4 c, PlaneAnd& s)
+__device__ __forceinline__ void corner_flags(float4 c, PlaneAcc& s)
 {
-  const bool lx = c.x <= -c.w;
-  const bool hx = !lx && (c.w <= c.x);
-  const bool ly = c.y <= -c.w;
-  const bool hy = !ly && (c.w <= c.y);
-  const bool lz = c.z <= -c.w;
-  const bool hz = !lz && (c.w <= c.z);
-  s.a0 = s.a0 && lx;
-  s.a1 = s.a1 && hx;
-  s.a2 = s.a2 && ly;
-  s.a3 = s.a3 && hy;
-  s.a4 = s.a4 && lz;
-  s.a5 = s.a5 && hz;
+  const float cv[3] = {c.x, c.y, c.z};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const bool lo = cv[a] <= -c.w;
+    const bool hi = c.w <= cv[a];
+    // bitwise (not short-circuit) operators: straight-line code, no divergent branches
+    s.lo_all[a] = s.lo_all[a] & lo;
+    s.lo_any[a] = s.lo_any[a] | lo;
+    s.hi_all[a] = s.hi_all[a] & hi;
+  }
 }
 
 // isVisible(projection, OBB) (ManagerImpl.cpp:263-289).  The reference returns
@@ -153,7 +156,7 @@ __device__ __forceinline__ bool obb_visible(const Obb& o, const float* vp)
   const float4 c5 = add4(c1, z);
   const float4 c6 = add4(c2, z);
   const float4 c7 = add4(c3, z);
-  PlaneAnd s{true, true, true, true, true, true};
+  PlaneAcc s{{true, true, true}, {false, false, false}, {true, true, true}};
   corner_flags(v0, s);
   corner_flags(c1, s);
   corner_flags(c2, s);
@@ -162,7 +165,48 @@ __device__ __forceinline__ bool obb_visible(const Obb& o, const float* vp)
   corner_flags(c5, s);
   corner_flags(c6, s);
   corner_flags(c7, s);
-  return !(s.a0 || s.a1 || s.a2 || s.a3 || s.a4 || s.a5);
+  bool culled = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) culled = culled | s.lo_all[a] | (s.hi_all[a] & !s.lo_any[a]);
+  return !culled;
 }
+
+// The raw inputs of one object as loaded from the SoA arrays: box (lower.xyz,
+// extent.x), (extent.y, extent.z) and the four matrix rows.  Loading and using
+// are separate steps so that kernels can issue the loads of the next tile
+// before they compute the current one.
+struct RawObj {
+  float4 a;
+  float2 b;
+  float4 r0, r1, r2, r3;
+};
+
+// Loads object i whose matrix index is m (m >= ld marks an index outside the
+// matrix array: nothing is read and the object becomes a NaN box, which sets
+// no cull flag, so it is reported visible).
+__device__ __forceinline__ RawObj load_raw(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
+                                           const float4* __restrict__ planes, size_t ld, size_t i, size_t m)
+{
+  RawObj r;
+  if (m < ld) {
+    r.a = box_a[i];
+    r.b = box_b[i];
+    r.r0 = planes[m];
+    r.r1 = planes[ld + m];
+    r.r2 = planes[2 * ld + m];
+    r.r3 = planes[3 * ld + m];
+  } else {
+    const float q = __builtin_nanf("");
+    r.a = make_float4(q, q, q, q);
+    r.b = make_float2(q, q);
+    r.r0 = r.a;
+    r.r1 = r.a;
+    r.r2 = r.a;
+    r.r3 = r.a;
+  }
+  return r;
+}
+
+__device__ __forceinline__ Obb obb_of(const RawObj& r) { return make_obb(r.a, r.b, r.r0, r.r1, r.r2, r.r3); }
 
 }  // namespace dpc

@@ -204,3 +204,48 @@ def cull(scene: DeviceScene, vp: np.ndarray, out: DeviceArray | None = None, str
     check(lib.dpc_cull(scene.box_a.ptr, scene.box_b.ptr, scene.xform_addr, scene.n_objects, scene.planes.ptr,
                        scene.ld, vp_arg(vp), out.ptr, stream))
     return out
+
+
+def scene_desc(scene: DeviceScene) -> "_lib.SceneDesc":
+    return _lib.SceneDesc(scene.box_a.addr, scene.box_b.addr, scene.xform_addr or None, scene.planes.addr,
+                          scene.ld, scene.n_objects)
+
+
+class BatchPlan:
+    """dpc_batch_plan: groups = (scene, first_object, count), pairs = (frustum, group, mask word offset)."""
+
+    def __init__(self, groups, pairs, n_frusta: int, cfg=None):
+        lib = _lib.load()
+        G, P = len(groups), len(pairs)
+        g = (_lib.GroupRange * max(G, 1))(*[_lib.GroupRange(int(s), 0, int(a), int(n)) for s, a, n in groups])
+        p = (_lib.CullPair * max(P, 1))(*[_lib.CullPair(int(f), int(gi), int(o)) for f, gi, o in pairs])
+        self.handle = C.c_void_p()
+        check(lib.dpc_batch_plan_create(g, G, p, P, int(n_frusta), C.byref(cfg) if cfg is not None else None,
+                                        C.byref(self.handle)))
+        mw, tests, items = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib.dpc_batch_plan_info(self.handle, C.byref(mw), C.byref(tests), C.byref(items)))
+        self.mask_words, self.tests, self.items = mw.value, tests.value, items.value
+
+    def bind(self, scenes) -> None:
+        table = (_lib.SceneDesc * max(len(scenes), 1))(*[scene_desc(s) for s in scenes])
+        check(_lib.load().dpc_batch_plan_bind(self.handle, table, len(scenes)))
+
+    def cull(self, d_vps: DeviceArray, d_masks: DeviceArray, stream=None) -> None:
+        check(_lib.load().dpc_cull_batch(self.handle, d_vps.ptr, d_masks.ptr, d_masks.nbytes // 8, stream))
+
+    def cull_per_view(self, h_vps: np.ndarray, d_masks: DeviceArray, stream=None) -> None:
+        self._h_vps = np.ascontiguousarray(h_vps, np.float32).reshape(-1)
+        check(_lib.load().dpc_cull_per_view(self.handle, self._h_vps.ctypes.data_as(C.POINTER(C.c_float)),
+                                            d_masks.ptr, d_masks.nbytes // 8, stream))
+
+    def destroy(self) -> None:
+        if self.handle.value:
+            _lib.load().dpc_batch_plan_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+
+def batched_plan(bs, cfg=None, scene_index: int = 0) -> BatchPlan:
+    """The plan of a scenes.BatchedScene whose objects form one device scene."""
+    groups = [(scene_index, a, n) for a, n in zip(bs.group_first, bs.group_count)]
+    pairs = list(zip(bs.pair_frustum, bs.pair_group, bs.pair_offset))
+    return BatchPlan(groups, pairs, bs.vps.shape[0], cfg)

@@ -203,6 +203,33 @@ def gen_tree(ref, rng):
         arrays[f"c{k}_levels"] = (np.concatenate(c["levels"]) if c["levels"] else np.zeros(0, np.uint32))
     arrays["n_computes"] = np.array(len(computes))
     np.savez_compressed(OUT / "ref_tree.npz", **arrays)
+    write_tree_trace_bin(arrays)
+
+
+def write_tree_trace_bin(arrays):
+    """The same trace as a flat little-endian file for the C++ tests
+    (tests/cpp/tree_trace.hpp reads it):
+      "DPTT" u32 version=1
+      u32 n_ops, then n_ops x (u32 kind, u32 a, u32 b, 16 x f32 matrix)
+      u32 n_computes, then per compute:
+        u32 n_levels, n_levels x u32 size, entries (2 x u32 each),
+        u32 n_world, n_world x 16 f32 world matrices, n_world x u8 event bytes"""
+    import struct
+    ops, mats = arrays["ops"], arrays["op_mats"]
+    out = bytearray(b"DPTT") + struct.pack("<I", 1)
+    out += struct.pack("<I", len(ops))
+    for (kind, a, b), m in zip(ops, mats):
+        out += struct.pack("<III", int(kind), int(a), int(b)) + np.asarray(m, "<f4").tobytes()
+    n_c = int(arrays["n_computes"])
+    out += struct.pack("<I", n_c)
+    for k in range(n_c):
+        lens = arrays[f"c{k}_level_len"]
+        out += struct.pack("<I", len(lens)) + np.asarray(lens, "<u4").tobytes()
+        out += np.asarray(arrays[f"c{k}_levels"], "<u4").tobytes()
+        w = arrays[f"c{k}_world"]
+        out += struct.pack("<I", w.shape[0]) + np.asarray(w, "<f4").tobytes()
+        out += np.asarray(arrays[f"c{k}_event"], np.uint8).tobytes()
+    (OUT / "ref_tree_trace.bin").write_bytes(bytes(out))
 
 
 def main():

@@ -1,0 +1,37 @@
+"""Runs the C++ tests of the host mirrors (tests/cpp): the transform tree's
+bookkeeping against the reference's Tree trace (CPU), the GPU tree, and the
+HIP culling manager against the CPU manager restatement (GPU)."""
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+BIN = ROOT / "tests" / "cpp" / "bin"
+
+
+def run(name, *args, timeout=300):
+    exe = BIN / name
+    if not exe.exists():
+        subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp")], check=True)
+    res = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=timeout, cwd=str(ROOT))
+    print(res.stdout[-4000:])
+    assert res.returncode == 0, res.stdout[-4000:] + res.stderr[-2000:]
+    return res.stdout
+
+
+def test_tree_layout_matches_reference_trace():
+    out = run("tree_layout_test", str(GOLDEN / "ref_tree_trace.bin"))
+    assert "0 failures" in out
+
+
+@pytest.mark.gpu
+def test_gpu_tree_matches_reference_trace():
+    out = run("tree_gpu_test", str(GOLDEN / "ref_tree_trace.bin"))
+    assert "tree gpu: 0 failures" in out
+
+
+@pytest.mark.gpu
+def test_manager_matches_cpu_manager():
+    out = run("manager_parity")
+    assert "manager parity: 0 failures" in out

@@ -29,41 +29,25 @@ def oracle_pairs(orc, bs: scenes.BatchedScene) -> np.ndarray:
 
 class Batch:
     def __init__(self, bs, cfg=None):
-        lib = _lib.load()
         self.bs = bs
         self.scene = dev.upload_scene(bs.lower, bs.upper, bs.matrices)
-        G = len(bs.group_first)
-        groups = (_lib.GroupRange * G)(*[_lib.GroupRange(int(a), int(n)) for a, n in zip(bs.group_first, bs.group_count)])
-        P = len(bs.pair_group)
-        pairs = (_lib.CullPair * P)(*[_lib.CullPair(int(f), int(g), int(o)) for f, g, o in
-                                      zip(bs.pair_frustum, bs.pair_group, bs.pair_offset)])
-        self.plan = C.c_void_p()
-        cfg_p = C.byref(cfg) if cfg is not None else None
-        _lib.check(lib.dpc_batch_plan_create(groups, G, pairs, P, bs.vps.shape[0], cfg_p, C.byref(self.plan)))
-        mw, tests, items = C.c_uint64(), C.c_uint64(), C.c_uint64()
-        _lib.check(lib.dpc_batch_plan_info(self.plan, C.byref(mw), C.byref(tests), C.byref(items)))
-        assert mw.value == bs.mask_words and tests.value == bs.tests
-        self.items = items.value
+        self.plan = dev.batched_plan(bs, cfg)
+        self.plan.bind([self.scene])
+        assert self.plan.mask_words == bs.mask_words and self.plan.tests == bs.tests
         self.vps = dev.DeviceArray.from_host(bs.vps.reshape(-1, 16))
         self.masks = dev.DeviceArray(max(bs.mask_words, 1), np.uint64)
 
     def run_batch(self, stream=None):
-        s = self.scene
-        _lib.check(_lib.load().dpc_cull_batch(self.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr,
-                                              s.ld, self.vps.ptr, self.masks.ptr, self.masks.shape[0], stream))
+        self.plan.cull(self.vps, self.masks, stream)
 
     def run_per_view(self, stream=None):
-        s = self.scene
-        h = np.ascontiguousarray(self.bs.vps.reshape(-1), F)
-        _lib.check(_lib.load().dpc_cull_per_view(self.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects,
-                                                 s.planes.ptr, s.ld, h.ctypes.data_as(C.POINTER(C.c_float)),
-                                                 self.masks.ptr, self.masks.shape[0], stream))
+        self.plan.cull_per_view(self.bs.vps, self.masks, stream)
 
     def result(self):
         return self.masks.download()
 
     def free(self):
-        _lib.load().dpc_batch_plan_destroy(self.plan)
+        self.plan.destroy()
         self.scene.free()
         self.vps.free()
         self.masks.free()
@@ -146,20 +130,58 @@ def test_graph_replay_with_moving_frusta(orc):
     b.free()
 
 
-def test_launch_rejects_short_buffers():
-    """Group ranges beyond the object arrays and short mask buffers are errors
-    reported before any launch."""
-    bs = small_scene(10)
-    b = Batch(bs)
-    s = b.scene
+def test_bind_and_launch_reject_bad_extents():
+    """Group ranges beyond a bound scene, an unbound plan and short mask buffers
+    are errors reported before any launch."""
+    import ctypes as C
     lib = _lib.load()
-    rc = lib.dpc_cull_batch(b.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects - 1, s.planes.ptr, s.ld,
-                            b.vps.ptr, b.masks.ptr, b.masks.shape[0], None)
-    assert rc == 1 and b"groups reach" in lib.dpc_last_error()
-    rc = lib.dpc_cull_batch(b.plan, s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr, s.ld,
-                            b.vps.ptr, b.masks.ptr, b.masks.shape[0] - 1, None)
+    bs = small_scene(10)
+    scene = dev.upload_scene(bs.lower, bs.upper, bs.matrices)
+    plan = dev.batched_plan(bs)
+    masks = dev.DeviceArray(bs.mask_words, np.uint64)
+    vps = dev.DeviceArray.from_host(bs.vps.reshape(-1, 16))
+    rc = lib.dpc_cull_batch(plan.handle, vps.ptr, masks.ptr, bs.mask_words, None)
+    assert rc == 6 and b"bound" in lib.dpc_last_error()
+    short = dev.scene_desc(scene)
+    short.n_objects = scene.n_objects - 1
+    table = (_lib.SceneDesc * 1)(short)
+    assert lib.dpc_batch_plan_bind(plan.handle, table, 1) == 1 and b"reaches object" in lib.dpc_last_error()
+    plan.bind([scene])
+    rc = lib.dpc_cull_batch(plan.handle, vps.ptr, masks.ptr, bs.mask_words - 1, None)
     assert rc == 1 and b"mask buffer" in lib.dpc_last_error()
-    b.free()
+    plan.destroy()
+    for d in (masks, vps):
+        d.free()
+    scene.free()
+
+
+def test_groups_across_scenes(orc):
+    """One launch over groups that live in different device scenes (each with its
+    own arrays), as the serving loop batches requests of resident scenes."""
+    a = small_scene(11)
+    b = small_scene(12)
+    sa = dev.upload_scene(a.lower, a.upper, a.matrices)
+    sb = dev.upload_scene(b.lower, b.upper, b.matrices)
+    vps = np.concatenate([a.vps, b.vps])
+    fa = a.vps.shape[0]
+    groups = [(0, f, n) for f, n in zip(a.group_first, a.group_count)] + \
+             [(1, f, n) for f, n in zip(b.group_first, b.group_count)]
+    ga = len(a.group_first)
+    pairs = list(zip(a.pair_frustum, a.pair_group, a.pair_offset)) + \
+        [(f + fa, g + ga, o + a.mask_words) for f, g, o in zip(b.pair_frustum, b.pair_group, b.pair_offset)]
+    plan = dev.BatchPlan(groups, pairs, vps.shape[0])
+    plan.bind([sa, sb])
+    d_vps = dev.DeviceArray.from_host(vps.reshape(-1, 16))
+    masks = dev.DeviceArray(plan.mask_words, np.uint64)
+    plan.cull(d_vps, masks)
+    got = masks.download()
+    want = np.concatenate([oracle_pairs(orc, a), oracle_pairs(orc, b)])
+    assert mismatches(got, want, plan.mask_words * 64) == 0
+    plan.destroy()
+    for d in (d_vps, masks):
+        d.free()
+    sa.free()
+    sb.free()
 
 
 def test_plan_rejects_bad_pairs():

@@ -12,6 +12,20 @@ F = np.float32
 ID = np.eye(4, dtype=F)
 
 
+@pytest.fixture(params=[0, 16, 3], ids=["grid", "stream16", "stream3"])
+def cull_shape(request):
+    """Runs a test under both launch shapes of dpc_cull (full grid; persistent
+    streaming grid, also with few waves so every wave walks many tiles)."""
+    import ctypes as C
+    from pipeline_amd import _lib
+    lib = _lib.load()
+    old = _lib.CullConfig()
+    _lib.check(lib.dpc_cull_get_config(C.byref(old)))
+    _lib.check(lib.dpc_cull_set_config(C.byref(_lib.CullConfig(request.param))))
+    yield request.param
+    _lib.check(lib.dpc_cull_set_config(C.byref(old)))
+
+
 def oracle_mask(orc, lower, upper, matrices, vp, xform=None):
     l4, e4 = scenes.extents(lower, upper)
     x = np.arange(lower.shape[0]) if xform is None else xform
@@ -26,7 +40,7 @@ def test_config0_flat_10k(orc):
     assert 0 < bits_of(want, s.n).sum() < s.n
 
 
-def test_config1_flat_1m(orc):
+def test_config1_flat_1m(orc, cull_shape):
     s = scenes.config1_scene()
     got = gpu_cull_flat(s.lower, s.upper, s.matrices, s.vp)
     want = oracle_mask(orc, s.lower, s.upper, s.matrices, s.vp)
@@ -36,13 +50,13 @@ def test_config1_flat_1m(orc):
 
 
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 127, 128, 129, 255, 256, 257, 1000, 4097])
-def test_ragged_sizes(orc, n):
+def test_ragged_sizes(orc, n, cull_shape):
     s = scenes.flat_scene(n, 100 + n, (-30.0, 30.0), 70.0, 1.3, 0.1, 60.0)
     got = gpu_cull_flat(s.lower, s.upper, s.matrices, s.vp)
     assert mismatches(got, oracle_mask(orc, s.lower, s.upper, s.matrices, s.vp), n) == 0
 
 
-def test_shared_matrices_through_indices(orc):
+def test_shared_matrices_through_indices(orc, cull_shape):
     """Objects reference matrices by transform index (many objects per matrix)."""
     rng = np.random.default_rng(5)
     n, nm = 50_000, 1_000
@@ -55,7 +69,7 @@ def test_shared_matrices_through_indices(orc):
     assert mismatches(got, want, n) == 0
 
 
-def test_out_of_range_index_stays_visible(orc):
+def test_out_of_range_index_stays_visible(orc, cull_shape):
     """A transform index beyond the matrix array is never read; the object is reported visible."""
     lower = np.array([[10, 10, 10], [10, 10, 10]], F)
     upper = lower + 1
@@ -96,7 +110,7 @@ def test_known_answer_cases_on_device(orc, case):
     assert mismatches(got, oracle_mask(orc, lo, up, m[None], vp), 1) == 0
 
 
-def test_special_values(orc):
+def test_special_values(orc, cull_shape):
     """Infinities, NaNs, denormals and signed zeros in boxes and matrices: the
     device evaluates them exactly as the oracle does (no flush-to-zero)."""
     rng = np.random.default_rng(9)
