@@ -14,6 +14,8 @@ Also measured in the same run and reported beside the headline:
             scene copies so the 256 MB last-level cache does not hold the input
   tree      1,000,000-node transform tree propagated level by level, then a
             cull of its ~2.4M leaf objects
+  serving   16 resident scenes, Poisson requests at 2k-20k req/s (1-32 frusta and a
+            1-10% matrix delta each), p50/p99 latency and tests/s per rate
   cpu       the CPU oracle (a restatement of dp::culling::cpu::ManagerImpl::cull)
             on one host core over a bounded sample of the batched workload
 
@@ -50,6 +52,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--skip-extra", action="store_true", help="only the batched and per-view lines")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of each serving-rate run")
     ap.add_argument("--tune", default="", help="batch config k,block,waves_per_cu,max_frusta")
     return ap.parse_args()
 
@@ -90,6 +93,136 @@ def run_timed(dev, lib, stream, steps, warmup, issue):
     e0.destroy()
     e1.destroy()
     return wall, dev_ms, dev_ms
+
+
+def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
+    """Serving loop: 16 resident scenes of log-uniform size in [10k, 2M] objects,
+    Poisson arrivals at each rate for --serve-seconds, every request = 1-32
+    random frusta + a delta of 1-10% of its scene's matrices.  Latency is the
+    server's submit -> masks-in-host-memory time.  The first rate's run is
+    checked against the oracle on sampled requests (host replay of every delta
+    of the sampled scenes in submission order)."""
+    import threading
+
+    from pipeline_amd import scenes
+    from pipeline_amd.serving import PinnedArray, Server
+
+    t0 = time.perf_counter()
+    scs = scenes.serving_scenes(seed=4, n_scenes=16)
+    sizes = [sc.n for sc in scs]
+    log(f"serving: 16 scenes, {sum(sizes)} objects (min {min(sizes)}, max {max(sizes)}), "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    # delta index and matrix pools in pinned memory: the uploads are DMA transfers
+    pinned = []
+    for sc in scs:
+        pp = PinnedArray(sc.perm.shape, np.uint32)
+        pp.array[:] = sc.perm
+        pm = PinnedArray(sc.pool.shape, np.float32)
+        pm.array[:] = sc.pool
+        pinned += [pp, pm]
+        sc.perm, sc.pool = pp.array, pm.array
+    n_slots, slot_words = 192, 32 * ((max(sizes) + 63) // 64)
+    out_pool = PinnedArray((n_slots, slot_words), np.uint64)
+    srv = Server(max_scenes=16, max_batch_requests=16, streams=4, hbm_budget_bytes=32 << 30)
+    ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scs]
+    base_mats = [sc.matrices.reshape(-1, 16).copy() for sc in scs]
+    runs = []
+    check = None
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(2e-4)  # the submit loop spins; let the collecting thread in often
+    for ri, rate in enumerate(rates):
+        n_req = max(int(rate * args.serve_seconds), 16)
+        reqs = scenes.serving_requests(scs, seed=100 + ri, n_requests=n_req, rate_per_s=float(rate))
+        sample = set()
+        if ri == 0:  # the last request of four scenes, checked against the oracle afterwards
+            last = {}
+            for i, r in enumerate(reqs):
+                last[r.scene] = i
+            order = sorted(last, key=lambda s: sizes[s])
+            sample = {last[s] for s in order[:2] + order[-2:]}
+        free = list(range(n_slots))
+        cv = threading.Condition()
+        ids_q = [None] * n_req
+        results = [None] * n_req
+        kept = {}
+
+        def waiter():
+            for i in range(n_req):
+                with cv:
+                    while ids_q[i] is None:
+                        cv.wait()
+                rid, slot = ids_q[i]
+                results[i] = srv.wait(rid)
+                if i in sample:
+                    kept[i] = out_pool.array[slot, :results[i].mask_words].copy()
+                with cv:
+                    free.append(slot)
+                    cv.notify_all()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        blocked_us = 0.0
+        start = srv.now_us() + 1000.0
+        for i, r in enumerate(reqs):
+            due = start + r.arrival_us
+            while srv.now_us() < due:
+                pass
+            with cv:
+                if not free:
+                    tb = srv.now_us()
+                    while not free:
+                        cv.wait()
+                    blocked_us += srv.now_us() - tb
+                slot = free.pop()
+            rid = srv.submit(ids[r.scene], r.vps, out_pool.array[slot], r.delta_idx, r.delta_mats)
+            with cv:
+                ids_q[i] = (rid, slot)
+                cv.notify_all()
+        t_submitted = srv.now_us()
+        th.join()
+        lat = np.array([x.latency_us for x in results])
+        t_first = min(x.submit_us for x in results)
+        t_last = max(x.done_us for x in results)
+        tests = sum(sizes[r.scene] * r.vps.shape[0] for r in reqs)
+        batches = len({x.batch for x in results})
+        runs.append(dict(offered_rate=rate, requests=n_req, achieved_submit_rate=n_req / ((t_submitted - start) * 1e-6),
+                         p50_us=float(np.percentile(lat, 50)), p99_us=float(np.percentile(lat, 99)),
+                         mean_us=float(lat.mean()), tests=tests,
+                         tests_per_s=tests / ((t_last - t_first) * 1e-6), batches=batches,
+                         mean_batch_requests=n_req / batches, submitter_blocked_us=blocked_us,
+                         delta_matrices=int(sum(r.delta_idx.size for r in reqs))))
+        log(f"serving @ {rate}/s: p50 {runs[-1]['p50_us']:.0f} us, p99 {runs[-1]['p99_us']:.0f} us, "
+            f"{runs[-1]['tests_per_s'] / 1e9:.1f} G tests/s, {batches} batches, "
+            f"submit rate {runs[-1]['achieved_submit_rate']:.0f}/s")
+        if ri == 0:
+            # oracle replay of the sampled scenes' deltas in submission order
+            mism, checked_tests = 0, 0
+            mats = {reqs[i].scene: base_mats[reqs[i].scene].copy() for i in sample}
+            for i, r in enumerate(reqs):
+                if r.scene not in mats:
+                    continue
+                mats[r.scene][r.delta_idx] = r.delta_mats.reshape(-1, 16)
+                if i in sample:
+                    sc = scs[r.scene]
+                    l4, e4 = scenes.extents(sc.lower, sc.upper)
+                    obbs = orc.obb_build_group(l4, e4, np.arange(sc.n), mats[r.scene])
+                    words = (sc.n + 63) // 64
+                    for f in range(min(4, r.vps.shape[0])):
+                        want = orc.cull_obbs(obbs, r.vps[f])
+                        got = kept[i][f * words:(f + 1) * words]
+                        mism += int(np.unpackbits((got ^ want).view(np.uint8)).sum())
+                        checked_tests += sc.n
+            check = dict(sampled_requests=len(sample), checked_tests=checked_tests, mismatches=mism)
+            log(f"serving oracle check: {check}")
+    sys.setswitchinterval(switch)
+    st = srv.stats()
+    srv.close()
+    out_pool.free()
+    for p in pinned:
+        p.free()
+    return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check,
+                server={"streams": 4, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
+                        "scene_bytes": st["scene_bytes"], "peak_inflight_bytes": st["peak_inflight_bytes"]})
 
 
 def main():
@@ -170,8 +303,8 @@ def main():
     value = tests / (wall_b / args.steps)
     per_view_value = tests / (wall_v / args.steps)
     achieved_tops = ops / (k_mean_b * 1e-3) / 1e12
-    log(f"batched: {k_mean_b * 1e3:.1f} us/launch (event), {wall_b / args.steps * 1e3:.1f} us/step wall, "
-        f"{value / 1e9:.2f} G tests/s; per-view {wall_v / args.steps * 1e3:.1f} us/step, "
+    log(f"batched: {k_mean_b * 1e3:.1f} us/launch (event), {wall_b / args.steps * 1e6:.1f} us/step wall, "
+        f"{value / 1e9:.2f} G tests/s; per-view {wall_v / args.steps * 1e6:.1f} us/step, "
         f"{per_view_value / 1e9:.2f} G tests/s; mismatches vs per-view {mism_pv}, vs oracle sample {mism_orc}")
     result["batched"] = dict(value=value, ms_per_step=wall_b / args.steps * 1e3, kernel_us=k_mean_b * 1e3,
                              kernel_us_median=k_med_b * 1e3, tests=tests, objects_referenced=referenced,
@@ -279,6 +412,9 @@ def main():
         for d in (d_local, d_world, d_parent, d_dirty, tmask):
             d.free()
         objs.free()
+
+        # ---------------- serving loop ----------------
+        result["serving"] = bench_serving(args, lib, orc)
 
     stream.destroy()
     peak_tops = PEAK_VALU_TOPS_PUBLISHED

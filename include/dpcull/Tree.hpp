@@ -114,6 +114,9 @@ class Tree {
   size_t deviceLd() const;
   const uint32_t* deviceSlotOfIndex() const;
   void* stream() const;
+  // Number of compute() calls so far; consumers that cache something derived
+  // from the world matrices compare it to know when to refresh.
+  uint64_t computeCount() const { return m_computeCount; }
 
  private:
   struct Device;
@@ -131,6 +134,7 @@ class Tree {
   std::vector<uint64_t> m_lastEvent;
   std::vector<TreeObserver*> m_observers;
   std::unique_ptr<Device> m_dev;
+  uint64_t m_computeCount = 0;
 };
 
 }  // namespace transform

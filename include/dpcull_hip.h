@@ -226,6 +226,73 @@ size_t dpc_bbox_workspace(size_t n_objects);
 int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
              const float* d_planes, size_t ld, float* d_out6, void* d_workspace, void* stream);
 
+/* ---- serving loop --------------------------------------------------------------
+ * No reference counterpart: the reference culls one scene per frame on the
+ * render thread (dp/sg/renderer/rix/gl/src/SceneRendererImpl.cpp:333-356).  A
+ * server keeps scenes resident on the device and runs a scheduler thread that
+ * batches cull requests of different scenes into one launch (continuous
+ * batching: requests that arrive while batches run join the next free stream).
+ * A request = apply a matrix delta to its scene, then cull the whole scene
+ * against 1..64 frusta.  Requests of one scene run in submission order, one per
+ * batch; requests are admitted while the device memory they need fits the
+ * budget left after the resident scenes. */
+typedef struct dpc_server dpc_server;
+
+typedef struct dpc_server_config {
+  uint32_t max_scenes;          /* resident scenes (<= 64) */
+  uint32_t max_batch_requests;  /* requests per launch (<= 64) */
+  uint32_t streams;             /* batches in flight at once (1..8) */
+  uint32_t objects_per_lane;    /* kernel shape: 1 or 2 */
+  uint32_t waves_per_cu;        /* kernel shape: persistent waves per CU */
+  uint32_t reserved;
+  uint64_t hbm_budget_bytes;    /* resident scenes + in-flight requests */
+} dpc_server_config;
+
+typedef struct dpc_request_info {
+  double submit_us;   /* all times: microseconds on the server's steady clock */
+  double admit_us;
+  double done_us;
+  uint32_t batch;     /* batch sequence number */
+  uint32_t batch_requests;
+  uint64_t mask_words; /* n_frusta x ceil(n_objects / 64) */
+  int32_t status;     /* DPC_OK or an error code */
+  uint32_t reserved;
+} dpc_request_info;
+
+typedef struct dpc_server_stats {
+  uint64_t requests_done;
+  uint64_t batches;
+  uint64_t tests;            /* object-frustum tests */
+  uint64_t delta_matrices;
+  uint64_t scene_bytes;      /* device bytes of resident scenes */
+  uint64_t peak_inflight_bytes;
+  double busy_us;            /* sum over batches of launch-to-completion time */
+} dpc_server_stats;
+
+int dpc_server_create(const dpc_server_config* cfg, dpc_server** server);
+int dpc_server_destroy(dpc_server* server);
+/* Uploads a scene (host arrays, copied): lower_upper n x 6, matrices n_matrices x 16
+ * (row-major), xform n indices or NULL (object i uses matrix i).
+ * DPC_ERR_OVER_BUDGET when the scene does not fit. */
+int dpc_server_add_scene(dpc_server* server, const float* lower_upper, uint64_t n_objects, const float* matrices,
+                         uint64_t n_matrices, const uint32_t* xform, uint32_t* scene_id);
+int dpc_server_remove_scene(dpc_server* server, uint32_t scene_id);
+/* Queues a request.  vps (n_frusta x 16), delta_idx (n_delta distinct matrix
+ * indices) and delta_mats (n_delta x 16) are read while the request is being
+ * launched; masks_out (at least n_frusta x ceil(n_objects / 64) words,
+ * frustum-major, the dp::util::BitArray layout) receives the result.  All four
+ * are borrowed until dpc_server_wait returns for the request; pinned host memory
+ * (dpc_host_alloc) makes the copies DMA transfers. */
+int dpc_server_submit(dpc_server* server, uint32_t scene_id, const float* vps, uint32_t n_frusta,
+                      const uint32_t* delta_idx, const float* delta_mats, uint32_t n_delta, uint64_t* masks_out,
+                      uint64_t mask_capacity_words, uint64_t* request_id);
+/* Waits for a request (timeout_ms < 0: no limit) and fills info.  The request is
+ * forgotten afterwards.  Returns the request's status; DPC_ERR_STATE on timeout. */
+int dpc_server_wait(dpc_server* server, uint64_t request_id, double timeout_ms, dpc_request_info* info);
+int dpc_server_stats_get(dpc_server* server, dpc_server_stats* stats);
+/* The server's clock (microseconds), to schedule arrivals against. */
+double dpc_server_now_us(dpc_server* server);
+
 #ifdef __cplusplus
 }
 #endif

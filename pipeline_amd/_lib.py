@@ -11,6 +11,12 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "lib" / "libdpcull_hip.so"
 
 DPC_OK = 0
+DPC_ERR_INVALID_ARGUMENT = 1
+DPC_ERR_HIP = 2
+DPC_ERR_OUT_OF_MEMORY = 3
+DPC_ERR_OVER_BUDGET = 4
+DPC_ERR_NOT_FOUND = 5
+DPC_ERR_STATE = 6
 
 
 class DpcError(RuntimeError):
@@ -31,6 +37,24 @@ class GroupRange(C.Structure):
 
 class CullPair(C.Structure):
     _fields_ = [("frustum", C.c_uint32), ("group", C.c_uint32), ("mask_word_offset", C.c_uint64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("max_scenes", C.c_uint32), ("max_batch_requests", C.c_uint32), ("streams", C.c_uint32),
+                ("objects_per_lane", C.c_uint32), ("waves_per_cu", C.c_uint32), ("reserved", C.c_uint32),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("submit_us", C.c_double), ("admit_us", C.c_double), ("done_us", C.c_double),
+                ("batch", C.c_uint32), ("batch_requests", C.c_uint32), ("mask_words", C.c_uint64),
+                ("status", C.c_int32), ("reserved", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("batches", C.c_uint64), ("tests", C.c_uint64),
+                ("delta_matrices", C.c_uint64), ("scene_bytes", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
+                ("busy_us", C.c_double)]
 
 
 class CullConfig(C.Structure):
@@ -97,6 +121,14 @@ _SIGNATURES = {
     "dpc_result_diff": (I, [P, P, SZ, P, P, I, P, P]),
     "dpc_bbox_workspace": (SZ, [SZ]),
     "dpc_bbox": (I, [P, P, P, SZ, P, SZ, P, P, P]),
+    "dpc_server_create": (I, [C.POINTER(ServerConfig), C.POINTER(P)]),
+    "dpc_server_destroy": (I, [P]),
+    "dpc_server_add_scene": (I, [P, P, U64, P, U64, P, C.POINTER(U32)]),
+    "dpc_server_remove_scene": (I, [P, U32]),
+    "dpc_server_submit": (I, [P, U32, P, U32, P, P, U32, P, U64, C.POINTER(U64)]),
+    "dpc_server_wait": (I, [P, U64, C.c_double, C.POINTER(RequestInfo)]),
+    "dpc_server_stats_get": (I, [P, C.POINTER(ServerStats)]),
+    "dpc_server_now_us": (C.c_double, [P]),
 }
 
 _lib: C.CDLL | None = None

@@ -21,6 +21,7 @@
 #include <numeric>
 #include <vector>
 
+#include "batch_types.hpp"
 #include "cull_math.hpp"
 #include "internal.hpp"
 
@@ -40,12 +41,6 @@ struct WorkItem {
 };
 static_assert(sizeof(WorkItem) == 32, "WorkItem layout");
 
-struct KPair {
-  uint32_t frustum;
-  uint32_t pad;
-  uint64_t mask_word;  // first mask word of the (frustum, group) result
-};
-static_assert(sizeof(KPair) == 16, "KPair layout");
 
 struct ViewTile {
   uint64_t first_object;
@@ -56,7 +51,7 @@ struct ViewTile {
 };
 static_assert(sizeof(ViewTile) == 32, "ViewTile layout");
 
-// Device copy of a dpc_scene_desc.
+// Device copy of a dpc_scene_desc (same layout).
 struct SceneDev {
   const float4* box_a;
   const float2* box_b;
@@ -65,12 +60,7 @@ struct SceneDev {
   uint64_t ld;
   uint64_t n_objects;
 };
-static_assert(sizeof(SceneDev) == 48, "SceneDev layout");
-
-__device__ __forceinline__ size_t matrix_index(const uint32_t* __restrict__ xform, size_t i)
-{
-  return xform ? static_cast<size_t>(xform[i]) : i;
-}
+static_assert(sizeof(SceneDev) == 48 && sizeof(SceneDev) == sizeof(dpc_scene_desc), "SceneDev layout");
 
 __device__ __forceinline__ void load_vp(const float* __restrict__ vps, uint32_t frustum, Mat16& vp)
 {
@@ -78,23 +68,24 @@ __device__ __forceinline__ void load_vp(const float* __restrict__ vps, uint32_t 
   for (int c = 0; c < 16; ++c) vp.m[c] = vps[16 * static_cast<size_t>(frustum) + c];
 }
 
-// The wave-uniform loop state is software-pipelined twice: the objects of the
-// next work item are loaded (vector memory) while the current item is tested,
-// and the next frustum matrix is loaded (scalar memory) while the current one is
-// in use.  Neither wait lands in front of the FP32 work.
 template <int K>
 __device__ __forceinline__ void load_item(const SceneDev* __restrict__ scenes, const WorkItem& item, uint32_t lane,
-                                          RawObj (&raw)[K])
+                                          Slot (&slot)[K], RawObj (&raw)[K])
 {
   const SceneDev sc = scenes[item.scene];
+  const size_t end = item.first_object + item.count;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const uint32_t j = lane + kWave * k;
-    const size_t i = item.first_object + j;
-    raw[k] = load_raw(sc.box_a, sc.box_b, sc.planes, sc.ld, i, j < item.count ? matrix_index(sc.xform, i) : ~size_t(0));
+    slot[k] = make_slot(item.first_object + lane + kWave * k, end, sc.xform, sc.ld);
+    raw[k] = load_raw(sc.box_a, sc.box_b, sc.planes, sc.ld, slot[k].i, slot[k].m);
   }
 }
 
+// The wave-uniform loop state is software-pipelined twice: the objects of the
+// next work item are loaded (vector memory) while the current item is tested,
+// and the next frustum matrix is loaded (scalar memory) while the current one is
+// in use.  Both prefetches are unconditional (indices clamped to the last item
+// or frustum), so no wait lands in front of the FP32 work.
 template <int K>
 __global__ __launch_bounds__(256) void cull_batch_kernel(const WorkItem* __restrict__ items, uint32_t n_items,
                                                          const KPair* __restrict__ pairs,
@@ -108,46 +99,141 @@ __global__ __launch_bounds__(256) void cull_batch_kernel(const WorkItem* __restr
   uint32_t it = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + threadIdx.x / kWave);
   if (it >= n_items) return;
   WorkItem item = items[it];
+  Slot slot[K];
   RawObj raw[K];
-  load_item<K>(scenes, item, lane, raw);
-  while (it < n_items) {
+  load_item<K>(scenes, item, lane, slot, raw);
+  for (;;) {
     const uint32_t nit = it + n_waves;
-    WorkItem next_item = item;
+    const WorkItem next_item = items[nit < n_items ? nit : it];
+    Slot next_slot[K];
     RawObj next_raw[K];
-    if (nit < n_items) {
-      next_item = items[nit];
-      load_item<K>(scenes, next_item, lane, next_raw);
-    }
+    load_item<K>(scenes, next_item, lane, next_slot, next_raw);
     Obb o[K];
-    bool valid[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      valid[k] = lane + kWave * k < item.count;
-      o[k] = obb_of(raw[k]);
-    }
+    for (int k = 0; k < K; ++k) o[k] = obb_of(raw[k]);
     KPair pr = pairs[item.pair_begin];
     Mat16 vp;
     load_vp(vps, pr.frustum, vp);
     for (uint32_t p = 0; p < item.pair_count; ++p) {
-      KPair next_pr = pr;
+      const uint32_t pn = p + 1 < item.pair_count ? p + 1 : p;
+      const KPair next_pr = pairs[item.pair_begin + pn];
       Mat16 next_vp;
-      if (p + 1 < item.pair_count) {
-        next_pr = pairs[item.pair_begin + p + 1];
-        load_vp(vps, next_pr.frustum, next_vp);
-      }
+      load_vp(vps, next_pr.frustum, next_vp);
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const bool vis = valid[k] & obb_visible(o[k], vp.m);
+        const bool vis = slot_visible(slot[k], obb_visible(o[k], vp.m));
         const unsigned long long bits = __ballot(vis);
         if (lane == 0 && static_cast<uint32_t>(kWave * k) < item.count) masks[pr.mask_word + item.tile_word + k] = bits;
       }
       pr = next_pr;
       vp = next_vp;
     }
+    if (nit >= n_items) break;
     it = nit;
     item = next_item;
 #pragma unroll
-    for (int k = 0; k < K; ++k) raw[k] = next_raw[k];
+    for (int k = 0; k < K; ++k) {
+      slot[k] = next_slot[k];
+      raw[k] = next_raw[k];
+    }
+  }
+}
+
+// ---- serving form -----------------------------------------------------------
+// One entry per request of a serving batch: the whole scene is tested against
+// the request's frusta (pairs[pair_begin .. +pair_count)).  Work is split
+// without a per-batch item list: a tile of entry a costs pair_count(a), every
+// wave takes the tiles whose cost prefix falls in its equal share of the total
+// cost, and walks them in order (the entry changes at most n_entries times).
+
+template <int K>
+__device__ __forceinline__ WorkItem serve_item(const ServeEntry& e, uint32_t tile)
+{
+  WorkItem w;
+  w.first_object = static_cast<uint64_t>(tile) * (kWave * K);
+  const uint64_t left = e.objects - w.first_object;
+  w.count = static_cast<uint32_t>(left < static_cast<uint64_t>(kWave * K) ? left : static_cast<uint64_t>(kWave * K));
+  w.tile_word = tile * K;
+  w.pair_begin = e.pair_begin;
+  w.pair_count = e.pair_count;
+  w.scene = e.scene;
+  w.pad = 0;
+  return w;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void cull_serve_kernel(const ServeEntry* __restrict__ entries, uint32_t n_entries,
+                                                         uint64_t total_cost, const KPair* __restrict__ pairs,
+                                                         const SceneDev* __restrict__ scenes,
+                                                         const float* __restrict__ vps,
+                                                         unsigned long long* __restrict__ masks)
+{
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint32_t waves_per_block = blockDim.x / kWave;
+  const uint64_t W = static_cast<uint64_t>(gridDim.x) * waves_per_block;
+  const uint64_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * waves_per_block + threadIdx.x / kWave);
+  const uint64_t c0 = total_cost * wave / W, c1 = total_cost * (wave + 1) / W;
+  if (c0 >= c1) return;
+  // first tile whose cost start is >= c0
+  uint32_t a = 0;
+  while (a + 1 < n_entries && entries[a + 1].cost_begin <= c0) ++a;
+  ServeEntry e = entries[a];
+  uint64_t t = e.pair_count ? (c0 - e.cost_begin + e.pair_count - 1) / e.pair_count : e.tiles;
+  auto advance = [&](ServeEntry& ent, uint32_t& idx, uint64_t& tile) {
+    while (tile >= ent.tiles && idx + 1 < n_entries) {
+      ++idx;
+      ent = entries[idx];
+      tile = 0;
+    }
+  };
+  advance(e, a, t);
+  auto cost_start = [](const ServeEntry& ent, uint64_t tile) { return ent.cost_begin + tile * ent.pair_count; };
+  if (t >= e.tiles || cost_start(e, t) >= c1) return;
+  WorkItem item = serve_item<K>(e, static_cast<uint32_t>(t));
+  Slot slot[K];
+  RawObj raw[K];
+  load_item<K>(scenes, item, lane, slot, raw);
+  for (;;) {
+    // the next tile of this wave (or the current one again, as a harmless prefetch target)
+    ServeEntry ne = e;
+    uint32_t na = a;
+    uint64_t nt = t + 1;
+    advance(ne, na, nt);
+    const bool more = nt < ne.tiles && cost_start(ne, nt) < c1;
+    const WorkItem next_item = more ? serve_item<K>(ne, static_cast<uint32_t>(nt)) : item;
+    Slot next_slot[K];
+    RawObj next_raw[K];
+    load_item<K>(scenes, next_item, lane, next_slot, next_raw);
+    Obb o[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) o[k] = obb_of(raw[k]);
+    KPair pr = pairs[item.pair_begin];
+    Mat16 vp;
+    load_vp(vps, pr.frustum, vp);
+    for (uint32_t p = 0; p < item.pair_count; ++p) {
+      const uint32_t pn = p + 1 < item.pair_count ? p + 1 : p;
+      const KPair next_pr = pairs[item.pair_begin + pn];
+      Mat16 next_vp;
+      load_vp(vps, next_pr.frustum, next_vp);
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const bool vis = slot_visible(slot[k], obb_visible(o[k], vp.m));
+        const unsigned long long bits = __ballot(vis);
+        if (lane == 0 && static_cast<uint32_t>(kWave * k) < item.count) masks[pr.mask_word + item.tile_word + k] = bits;
+      }
+      pr = next_pr;
+      vp = next_vp;
+    }
+    if (!more) break;
+    e = ne;
+    a = na;
+    t = nt;
+    item = next_item;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      slot[k] = next_slot[k];
+      raw[k] = next_raw[k];
+    }
   }
 }
 
@@ -160,12 +246,9 @@ __global__ __launch_bounds__(256) void cull_tiles_kernel(const ViewTile* __restr
   const uint32_t t = __builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave);
   if (t >= n_tiles) return;
   const ViewTile tile = tiles[t];
-  bool vis = false;
-  if (lane < tile.count) {
-    const SceneDev sc = scenes[tile.scene];
-    const size_t i = tile.first_object + lane;
-    vis = obb_visible(obb_of(load_raw(sc.box_a, sc.box_b, sc.planes, sc.ld, i, matrix_index(sc.xform, i))), vp.m);
-  }
+  const SceneDev sc = scenes[tile.scene];
+  const Slot sl = make_slot(tile.first_object + lane, tile.first_object + tile.count, sc.xform, sc.ld);
+  const bool vis = slot_visible(sl, obb_visible(obb_of(load_raw(sc.box_a, sc.box_b, sc.planes, sc.ld, sl.i, sl.m)), vp.m));
   const unsigned long long bits = __ballot(vis);
   if (lane == 0) masks[tile.mask_word] = bits;
 }
@@ -462,6 +545,41 @@ int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_vps, uint64_t* d_m
   }
   return check_launch("cull_batch_kernel");
 }
+
+}  // extern "C"
+
+namespace dpc {
+// Launches one serving batch (see ServeEntry).  d_entries/d_pairs/d_vps/d_masks
+// live on the device; total_cost and n_entries come from the host copy.
+int launch_serve(const void* d_entries, uint32_t n_entries, uint64_t total_cost, const void* d_pairs,
+                 const void* d_scenes, const float* d_vps, uint64_t* d_masks, uint32_t objects_per_lane,
+                 uint32_t waves_per_cu, void* stream_handle)
+{
+  hipStream_t stream = as_stream(stream_handle);
+  if (n_entries == 0 || total_cost == 0) return DPC_OK;
+  const uint32_t waves_per_block = 4;
+  uint64_t waves = static_cast<uint64_t>(device_cu_count()) * (waves_per_cu ? waves_per_cu : 24);
+  if (waves > total_cost) waves = total_cost;
+  const uint64_t blocks = (waves + waves_per_block - 1) / waves_per_block;
+  const auto* e = static_cast<const ServeEntry*>(d_entries);
+  const auto* p = static_cast<const KPair*>(d_pairs);
+  const auto* sc = static_cast<const SceneDev*>(d_scenes);
+  auto* m = reinterpret_cast<unsigned long long*>(d_masks);
+  switch (objects_per_lane) {
+    case 2:
+      hipLaunchKernelGGL(cull_serve_kernel<2>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, e, n_entries,
+                         total_cost, p, sc, d_vps, m);
+      break;
+    default:
+      hipLaunchKernelGGL(cull_serve_kernel<1>, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, stream, e, n_entries,
+                         total_cost, p, sc, d_vps, m);
+      break;
+  }
+  return check_launch("cull_serve_kernel");
+}
+}  // namespace dpc
+
+extern "C" {
 
 int dpc_cull_per_view(const dpc_batch_plan* plan, const float* h_vps, uint64_t* d_masks, size_t mask_words,
                       void* stream)

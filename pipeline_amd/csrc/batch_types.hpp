@@ -1,0 +1,35 @@
+// batch_types.hpp -- device-visible descriptor layouts shared by the batched
+// kernels (batch.hip) and the host code that fills them (host/Serving.cpp).
+#pragma once
+#include <cstdint>
+
+namespace dpc {
+
+// One (frustum, group) pair of a batch: the frustum's index in the batch's
+// matrix array and the first 64-bit mask word of its result.
+struct KPair {
+  uint32_t frustum;
+  uint32_t pad;
+  uint64_t mask_word;
+};
+static_assert(sizeof(KPair) == 16, "KPair layout");
+
+// One request of a serving batch: the whole scene against pairs
+// [pair_begin, pair_begin + pair_count).  cost_begin = sum of tiles * pair_count
+// over the earlier entries (the kernel splits the total cost evenly over waves).
+struct ServeEntry {
+  uint32_t scene;
+  uint32_t pair_begin;
+  uint32_t pair_count;
+  uint32_t tiles;  // ceil(objects / (64 * objects_per_lane))
+  uint64_t cost_begin;
+  uint64_t objects;
+};
+static_assert(sizeof(ServeEntry) == 32, "ServeEntry layout");
+
+// Launches one serving batch; every pointer is device memory.
+int launch_serve(const void* d_entries, uint32_t n_entries, uint64_t total_cost, const void* d_pairs,
+                 const void* d_scenes, const float* d_vps, uint64_t* d_masks, uint32_t objects_per_lane,
+                 uint32_t waves_per_cu, void* stream);
+
+}  // namespace dpc

@@ -30,20 +30,39 @@ __global__ __launch_bounds__(kCullThreads) void cull_kernel(const float4* __rest
                                                             unsigned long long* __restrict__ mask)
 {
   const size_t i = static_cast<size_t>(blockIdx.x) * kCullThreads + threadIdx.x;
-  bool vis = false;
-  if (i < n) {
-    const size_t m = Indexed ? static_cast<size_t>(xform[i]) : i;
-    vis = obb_visible(obb_of(load_raw(box_a, box_b, planes, ld, i, m)), vp.m);
-  }
+  const Slot s = make_slot(i, n, Indexed ? xform : nullptr, ld);
+  const RawObj r = load_raw(box_a, box_b, planes, ld, s.i, s.m);
+  const bool vis = slot_visible(s, obb_visible(obb_of(r), vp.m));
   const unsigned long long bits = __ballot(vis);
   if ((threadIdx.x & (kWave - 1)) == 0 && i < n) mask[i >> 6] = bits;
 }
 
 // Streaming form: a persistent grid in which every wave walks tiles t, t+W,
-// t+2W, ... and issues the loads of its next tile before it computes the
-// current one, so memory and the FP32 pipeline stay busy together.  With
-// indices, the index of the tile after next is loaded one step earlier still,
-// so the dependent row gather never waits inside the loop.
+// t+2W, ... with two register sets: while one tile is tested, the loads of the
+// next one are in flight.  Every wave runs the same number of steps (tiles past
+// the end load clamped indices and store nothing), so all loads are
+// unconditional and the wait before each compute covers only that tile's loads.
+template <bool Indexed>
+__device__ __forceinline__ void stream_fetch(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
+                                             const uint32_t* __restrict__ xform, size_t n,
+                                             const float4* __restrict__ planes, size_t ld, size_t tile, unsigned lane,
+                                             Slot& s, RawObj& r)
+{
+  s = make_slot(tile * kWave + lane, n, Indexed ? xform : nullptr, ld);
+  r = load_raw(box_a, box_b, planes, ld, s.i, s.m);
+  // keep the loads issued here, ahead of the test of the previous tile (the
+  // scheduler would otherwise sink them below it to shorten register lifetimes)
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+__device__ __forceinline__ void stream_test(const Slot& s, const RawObj& r, const Mat16& vp, size_t tile,
+                                            size_t n_tiles, unsigned lane, unsigned long long* __restrict__ mask)
+{
+  const bool vis = slot_visible(s, obb_visible(obb_of(r), vp.m));
+  const unsigned long long bits = __ballot(vis);
+  if (lane == 0 && tile < n_tiles) mask[tile] = bits;
+}
+
 template <bool Indexed>
 __global__ __launch_bounds__(kCullThreads) void cull_stream_kernel(const float4* __restrict__ box_a,
                                                                    const float2* __restrict__ box_b,
@@ -55,29 +74,18 @@ __global__ __launch_bounds__(kCullThreads) void cull_stream_kernel(const float4*
   const size_t waves_per_block = blockDim.x / kWave;
   const size_t W = static_cast<size_t>(gridDim.x) * waves_per_block;
   const size_t n_tiles = (n + kWave - 1) / kWave;
+  const size_t steps = (n_tiles + W - 1) / W;  // the same for every wave
   size_t t = static_cast<size_t>(blockIdx.x) * waves_per_block + threadIdx.x / kWave;
-  if (t >= n_tiles) return;
-  // matrix index of this lane's object in a tile (n: no object)
-  auto index_of = [&](size_t tile) -> size_t {
-    const size_t i = tile * kWave + lane;
-    if (tile >= n_tiles || i >= n) return ~size_t(0);
-    return Indexed ? static_cast<size_t>(xform[i]) : i;
-  };
-  size_t m_cur = index_of(t);
-  size_t m_next = index_of(t + W);
-  RawObj cur = load_raw(box_a, box_b, planes, ld, t * kWave + lane, m_cur);
-  while (t < n_tiles) {
-    const size_t tn = t + W;
-    const size_t m_after = Indexed ? index_of(tn + W) : ~size_t(0);
-    RawObj nxt;
-    if (tn < n_tiles) nxt = load_raw(box_a, box_b, planes, ld, tn * kWave + lane, m_next);
-    const size_t i = t * kWave + lane;
-    const bool vis = i < n && obb_visible(obb_of(cur), vp.m);
-    const unsigned long long bits = __ballot(vis);
-    if (lane == 0) mask[t] = bits;
-    cur = nxt;
-    m_next = Indexed ? m_after : index_of(tn + W);
-    t = tn;
+  Slot sa, sb;
+  RawObj ra, rb;
+  stream_fetch<Indexed>(box_a, box_b, xform, n, planes, ld, t, lane, sa, ra);
+  for (size_t k = 0; k < steps; k += 2) {
+    stream_fetch<Indexed>(box_a, box_b, xform, n, planes, ld, t + W, lane, sb, rb);
+    stream_test(sa, ra, vp, t, n_tiles, lane, mask);
+    if (k + 1 >= steps) break;
+    stream_fetch<Indexed>(box_a, box_b, xform, n, planes, ld, t + 2 * W, lane, sa, ra);
+    stream_test(sb, rb, vp, t + W, n_tiles, lane, mask);
+    t += 2 * W;
   }
 }
 
@@ -396,6 +404,7 @@ int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
   if (!d_xform && ld < n_objects)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: %zu objects without indices need ld >= n (ld=%zu)",
                      n_objects, ld);
+  if (ld == 0) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: no matrices (ld = 0)");
   Mat16 m;
   for (int k = 0; k < 16; ++k) m.m[k] = vp[k];
   const auto* a = reinterpret_cast<const float4*>(d_box_a);

@@ -181,31 +181,46 @@ struct RawObj {
   float4 r0, r1, r2, r3;
 };
 
-// Loads object i whose matrix index is m (m >= ld marks an index outside the
-// matrix array: nothing is read and the object becomes a NaN box, which sets
-// no cull flag, so it is reported visible).
+// Loads object i with matrix m.  The loads are unconditional so that no branch
+// separates them from the compute (which lets the compiler wait for exactly
+// the loads a tile needs): the caller passes indices that are in range, and
+// keeps its own per-lane flags for lanes without an object or with a matrix
+// index outside the array.
 __device__ __forceinline__ RawObj load_raw(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
                                            const float4* __restrict__ planes, size_t ld, size_t i, size_t m)
 {
   RawObj r;
-  if (m < ld) {
-    r.a = box_a[i];
-    r.b = box_b[i];
-    r.r0 = planes[m];
-    r.r1 = planes[ld + m];
-    r.r2 = planes[2 * ld + m];
-    r.r3 = planes[3 * ld + m];
-  } else {
-    const float q = __builtin_nanf("");
-    r.a = make_float4(q, q, q, q);
-    r.b = make_float2(q, q);
-    r.r0 = r.a;
-    r.r1 = r.a;
-    r.r2 = r.a;
-    r.r3 = r.a;
-  }
+  r.a = box_a[i];
+  r.b = box_b[i];
+  r.r0 = planes[m];
+  r.r1 = planes[ld + m];
+  r.r2 = planes[2 * ld + m];
+  r.r3 = planes[3 * ld + m];
   return r;
 }
+
+// Per-lane object slot of a tile: the (clamped) indices to load from and what
+// the lane's visibility bit becomes.  An object whose matrix index lies outside
+// the matrix array reads nothing of it and is reported visible (the value the
+// reference's test gives an object whose clip coordinates are NaN: no flag set).
+struct Slot {
+  size_t i, m;
+  bool valid;       // the lane holds an object
+  bool bad_matrix;  // its matrix index is outside the array
+};
+
+__device__ __forceinline__ Slot make_slot(size_t i, size_t n, const uint32_t* __restrict__ xform, size_t ld)
+{
+  Slot s;
+  s.valid = i < n;
+  s.i = s.valid ? i : 0;
+  const size_t m = xform ? static_cast<size_t>(xform[s.i]) : s.i;
+  s.bad_matrix = m >= ld;
+  s.m = s.bad_matrix ? 0 : m;
+  return s;
+}
+
+__device__ __forceinline__ bool slot_visible(const Slot& s, bool test) { return s.valid & (s.bad_matrix | test); }
 
 __device__ __forceinline__ Obb obb_of(const RawObj& r) { return make_obb(r.a, r.b, r.r0, r.r1, r.r2, r.r3); }
 

@@ -88,6 +88,7 @@ class GroupImpl : public Group, public std::enable_shared_from_this<GroupImpl> {
   size_t m_incarnationOBB = kNone;  // GroupCPU::m_objectIncarnationOBB
   bool m_objectsDirty = true;       // device object arrays out of date
   bool m_boundingBoxDirty = true;
+  uint64_t m_boundingBoxTreeCompute = 0;
   Box3f m_boundingBox;
   std::vector<ResultImpl*> m_results;
   // matrices from a GPU tree instead of the host pointer
@@ -299,9 +300,12 @@ class ManagerImpl : public culling::Manager {
   Box3f getBoundingBox(GroupSharedPtr const& group) const override
   {
     auto g = const_cast<ManagerImpl*>(this)->impl(group);
+    // a tree-bound group's world matrices change with every Tree::compute()
+    if (g->m_tree && g->m_boundingBoxTreeCompute != g->m_tree->computeCount()) g->m_boundingBoxDirty = true;
     if (g->m_boundingBoxDirty) {
       const_cast<ManagerImpl*>(this)->computeBoundingBox(*g);
       g->m_boundingBoxDirty = false;
+      if (g->m_tree) g->m_boundingBoxTreeCompute = g->m_tree->computeCount();
     }
     return g->m_boundingBox;
   }

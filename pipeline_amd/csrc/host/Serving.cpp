@@ -1,0 +1,556 @@
+// Serving.cpp -- a request loop over resident scenes (dpc_server_* in
+// include/dpcull_hip.h).
+//
+// Scenes are uploaded once and stay in device memory.  Callers submit requests
+// (scene, frusta, matrix delta) from any thread; a scheduler thread forms
+// batches continuously: whenever one of its streams is free it takes, in
+// arrival order, at most one request per scene (requests of one scene keep
+// their order: each sees the deltas of the earlier ones) whose scene is not in
+// a batch still running, until the batch's device arena is full.  A batch
+// copies the deltas to the device and scatters them into the scenes' matrices,
+// then tests every scene of the batch against its request's frusta in one
+// launch (dpc::launch_serve), then copies the masks back.  Batches on different
+// streams overlap on the device.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../batch_types.hpp"
+#include "dpcull_hip.h"
+
+namespace dpc {
+int set_error(int code, const char* fmt, ...);  // runtime.hip: the thread's dpc_last_error() text
+}
+
+namespace {
+
+constexpr uint32_t kMaxScenes = 64;
+constexpr uint32_t kMaxBatch = 64;
+constexpr uint32_t kMaxFrusta = 64;
+// the pinned staging area of one batch's descriptors (entries, pairs, frusta)
+constexpr uint64_t kHostArena = 4096 + kMaxBatch * 32 + kMaxBatch * kMaxFrusta * (16 + 64) + 3 * 256;
+
+uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+struct Scene {
+  bool live = false;
+  bool busy = false;  // in a batch that has not completed
+  uint64_t n = 0, n_mat = 0, bytes = 0;
+  float* box_a = nullptr;
+  float* box_b = nullptr;
+  float* planes = nullptr;
+  uint32_t* xform = nullptr;
+  uint32_t pending = 0;  // queued requests
+};
+
+struct Request {
+  uint64_t id = 0;
+  uint32_t scene = 0;
+  const float* vps = nullptr;
+  uint32_t n_frusta = 0;
+  const uint32_t* didx = nullptr;
+  const float* dmats = nullptr;
+  uint32_t n_delta = 0;
+  uint64_t* masks_out = nullptr;
+  uint64_t words = 0;       // mask words per frustum
+  uint64_t dev_bytes = 0;   // arena bytes this request needs
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  uint32_t batch = 0, batch_requests = 0;
+  int status = DPC_OK;
+  bool done = false;
+};
+
+struct StreamSlot {
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  char* d_arena = nullptr;
+  char* h_arena = nullptr;  // pinned, kHostArena bytes
+  uint64_t cap = 0;
+  bool busy = false;
+  std::vector<std::shared_ptr<Request>> reqs;
+  double t_launch = 0;
+  uint64_t bytes = 0;
+  uint64_t tests = 0;
+};
+
+}  // namespace
+
+struct dpc_server {
+  dpc_server_config cfg{};
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  std::mutex mu;
+  std::condition_variable cv_done;
+  std::condition_variable cv_work;
+  std::deque<std::shared_ptr<Request>> pending;
+  std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
+  std::vector<Scene> scenes;
+  dpc_scene_desc* d_scene_table = nullptr;
+  std::vector<StreamSlot> slots;
+  uint64_t next_id = 1;
+  uint32_t batch_seq = 0;
+  uint64_t scene_bytes = 0;
+  uint64_t arena_total = 0;
+  bool stop = false;
+  std::thread worker;
+  dpc_server_stats stats{};
+  int device = 0;
+
+  double now() const
+  {
+    return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+  }
+  void run();
+  bool launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lock);
+  void complete(StreamSlot& slot);
+};
+
+namespace {
+
+int fail(int code, const std::string& msg) { return dpc::set_error(code, "%s", msg.c_str()); }
+
+int hip_fail(hipError_t e, const char* what)
+{
+  return fail(DPC_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+}  // namespace
+
+// Forms and launches one batch on `slot`.  Called with the lock held; the HIP
+// calls run with it released (only the scheduler thread touches slots).
+bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lock)
+{
+  std::vector<std::shared_ptr<Request>> pick;
+  std::vector<bool> scene_taken(scenes.size(), false);
+  uint64_t bytes = 0;
+  for (auto it = pending.begin(); it != pending.end() && pick.size() < cfg.max_batch_requests;) {
+    const auto& r = *it;
+    Scene& sc = scenes[r->scene];
+    if (sc.busy || scene_taken[r->scene]) {
+      scene_taken[r->scene] = true;  // later requests of this scene wait behind this one
+      ++it;
+      continue;
+    }
+    if (bytes + r->dev_bytes > slot.cap) break;  // admission: the arena (memory budget) is full
+    scene_taken[r->scene] = true;
+    bytes += r->dev_bytes;
+    pick.push_back(r);
+    it = pending.erase(it);
+  }
+  if (pick.empty()) return false;
+  const uint32_t batch = ++batch_seq;
+  const double t_admit = now();
+  for (auto& r : pick) {
+    scenes[r->scene].busy = true;
+    scenes[r->scene].pending--;
+    r->t_admit = t_admit;
+    r->batch = batch;
+    r->batch_requests = static_cast<uint32_t>(pick.size());
+  }
+  std::vector<Scene> snapshot;  // scene arrays are stable while busy
+  snapshot.reserve(pick.size());
+  for (auto& r : pick) snapshot.push_back(scenes[r->scene]);
+  lock.unlock();
+
+  // Arena layout: the batch descriptors (entries, pairs, frusta; staged in the
+  // pinned host arena at the same offsets), then per request the delta indices
+  // and matrices, then the masks.
+  const uint32_t K = cfg.objects_per_lane;
+  hipStream_t s = slot.stream;
+  hipError_t e = hipSuccess;
+  uint32_t n_pairs = 0;
+  for (auto& r : pick) n_pairs += r->n_frusta;
+  uint64_t off = 0;
+  const uint64_t entries_off = off;
+  off = align_up(off + sizeof(dpc::ServeEntry) * pick.size(), 256);
+  const uint64_t pairs_off = off;
+  off = align_up(off + sizeof(dpc::KPair) * n_pairs, 256);
+  const uint64_t vps_off = off;
+  off = align_up(off + 64ull * n_pairs, 256);
+  const uint64_t desc_end = off;
+  std::vector<uint64_t> idx_off(pick.size()), mat_off(pick.size());
+  for (size_t k = 0; k < pick.size(); ++k) {
+    const Request& r = *pick[k];
+    idx_off[k] = off;
+    off = align_up(off + 4ull * r.n_delta, 256);
+    mat_off[k] = off;
+    off = align_up(off + 64ull * r.n_delta, 256);
+  }
+  const uint64_t masks_off = off;
+  auto* entries = reinterpret_cast<dpc::ServeEntry*>(slot.h_arena + entries_off);
+  auto* pairs = reinterpret_cast<dpc::KPair*>(slot.h_arena + pairs_off);
+  auto* vps = reinterpret_cast<float*>(slot.h_arena + vps_off);
+  std::vector<uint64_t> mask_off(pick.size());  // byte offset of each request's masks in the device arena
+  uint64_t cost = 0, mask_word = 0, tests = 0;
+  uint32_t p = 0;
+  for (size_t k = 0; k < pick.size(); ++k) {
+    const Request& r = *pick[k];
+    const Scene& sc = snapshot[k];
+    dpc::ServeEntry& en = entries[k];
+    en.scene = r.scene;
+    en.pair_begin = p;
+    en.pair_count = r.n_frusta;
+    en.tiles = static_cast<uint32_t>((sc.n + 64ull * K - 1) / (64ull * K));
+    en.cost_begin = cost;
+    en.objects = sc.n;
+    cost += static_cast<uint64_t>(en.tiles) * r.n_frusta;
+    mask_off[k] = masks_off + 8 * mask_word;
+    for (uint32_t f = 0; f < r.n_frusta; ++f, ++p) {
+      pairs[p].frustum = p;
+      pairs[p].pad = 0;
+      pairs[p].mask_word = mask_word;
+      mask_word += r.words;
+      std::memcpy(vps + 16ull * p, r.vps + 16ull * f, 64);
+    }
+    tests += sc.n * r.n_frusta;
+  }
+  const uint64_t end = masks_off + 8 * mask_word;
+  int rc = DPC_OK;
+  if (end > slot.cap) rc = fail(DPC_ERR_OVER_BUDGET, "batch larger than its arena");
+  // deltas: copy to the device and scatter into the scenes' matrices
+  for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
+    const Request& r = *pick[k];
+    if (!r.n_delta) continue;
+    e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(slot.d_arena + mat_off[k], r.dmats, 64ull * r.n_delta, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "delta upload");
+      break;
+    }
+    rc = dpc_matrices_scatter(snapshot[k].planes, snapshot[k].n_mat,
+                              reinterpret_cast<const uint32_t*>(slot.d_arena + idx_off[k]),
+                              reinterpret_cast<const float*>(slot.d_arena + mat_off[k]), r.n_delta, s);
+  }
+  if (rc == DPC_OK) {
+    e = hipMemcpyAsync(slot.d_arena, slot.h_arena, desc_end, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) rc = hip_fail(e, "descriptor upload");
+  }
+  if (rc == DPC_OK) {
+    rc = dpc::launch_serve(slot.d_arena + entries_off, static_cast<uint32_t>(pick.size()), cost,
+                           slot.d_arena + pairs_off, d_scene_table, reinterpret_cast<const float*>(slot.d_arena + vps_off),
+                           reinterpret_cast<uint64_t*>(slot.d_arena + masks_off), K, cfg.waves_per_cu, s);
+  }
+  // each request's masks go straight to the caller's buffer
+  for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
+    const Request& r = *pick[k];
+    e = hipMemcpyAsync(r.masks_out, slot.d_arena + mask_off[k], 8 * r.words * r.n_frusta, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) rc = hip_fail(e, "mask download");
+  }
+  if (rc == DPC_OK) {
+    e = hipEventRecord(slot.done, s);
+    if (e != hipSuccess) rc = hip_fail(e, "completion event");
+  }
+  lock.lock();
+  slot.reqs = pick;
+  slot.busy = true;
+  slot.t_launch = t_admit;
+  slot.bytes = end;
+  slot.tests = tests;
+  stats.peak_inflight_bytes = std::max<uint64_t>(stats.peak_inflight_bytes, end);
+  if (rc != DPC_OK) {
+    // nothing reads the arena; finish the batch as failed once the stream drains
+    (void)hipStreamSynchronize(s);
+    for (auto& r : pick) r->status = rc;
+    complete(slot);
+  }
+  return true;
+}
+
+// Called with the lock held once the slot's completion event has fired.
+void dpc_server::complete(StreamSlot& slot)
+{
+  const double t = now();
+  for (size_t k = 0; k < slot.reqs.size(); ++k) {
+    Request& r = *slot.reqs[k];
+    if (r.status == DPC_OK) stats.delta_matrices += r.n_delta;
+    r.t_done = t;
+    r.done = true;
+    scenes[r.scene].busy = false;
+    stats.requests_done++;
+  }
+  stats.batches++;
+  stats.tests += slot.tests;
+  stats.busy_us += t - slot.t_launch;
+  slot.reqs.clear();
+  slot.busy = false;
+  cv_done.notify_all();
+}
+
+void dpc_server::run()
+{
+  (void)hipSetDevice(device);
+  std::unique_lock<std::mutex> lock(mu);
+  for (;;) {
+    bool progressed = false;
+    bool any_busy = false;
+    for (auto& slot : slots) {
+      if (slot.busy) {
+        if (hipEventQuery(slot.done) == hipSuccess) {
+          complete(slot);
+          progressed = true;
+        } else {
+          any_busy = true;
+        }
+      }
+    }
+    for (auto& slot : slots) {
+      if (!slot.busy && !pending.empty() && launch_batch(slot, lock)) {
+        progressed = true;
+        any_busy = true;
+      }
+    }
+    if (stop && pending.empty() && !any_busy) break;
+    if (progressed) continue;
+    if (any_busy) {
+      // a batch is running: poll its event without sleeping (latency matters)
+      lock.unlock();
+      std::this_thread::yield();
+      lock.lock();
+    } else {
+      cv_work.wait_for(lock, std::chrono::microseconds(200));
+    }
+  }
+}
+
+extern "C" {
+
+int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
+{
+  if (!cfg || !out) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_create: null argument");
+  *out = nullptr;
+  if (cfg->max_scenes == 0 || cfg->max_scenes > kMaxScenes || cfg->max_batch_requests == 0 ||
+      cfg->max_batch_requests > kMaxBatch || cfg->streams == 0 || cfg->streams > 8 ||
+      (cfg->objects_per_lane != 1 && cfg->objects_per_lane != 2) || cfg->hbm_budget_bytes < (64ull << 20))
+    return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_create: bad configuration");
+  auto srv = std::make_unique<dpc_server>();
+  srv->cfg = *cfg;
+  (void)hipGetDevice(&srv->device);
+  srv->scenes.resize(cfg->max_scenes);
+  // a quarter of the budget (at most 1 GiB) is the in-flight request arena, split
+  // over the streams; each stream has a device arena and a pinned host twin
+  srv->arena_total = std::min<uint64_t>(cfg->hbm_budget_bytes / 4, 1ull << 30);
+  const uint64_t per = align_up(srv->arena_total / cfg->streams, 4096);
+  hipError_t e = hipMalloc(&srv->d_scene_table, sizeof(dpc_scene_desc) * cfg->max_scenes);
+  if (e != hipSuccess) return hip_fail(e, "scene table");
+  std::vector<dpc_scene_desc> zero(cfg->max_scenes, dpc_scene_desc{});
+  e = hipMemcpy(srv->d_scene_table, zero.data(), sizeof(dpc_scene_desc) * cfg->max_scenes, hipMemcpyHostToDevice);
+  srv->slots.resize(cfg->streams);
+  for (auto& sl : srv->slots) {
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sl.d_arena), per);
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_arena), kHostArena, hipHostMallocDefault);
+    sl.cap = per;
+  }
+  if (e != hipSuccess) {
+    int rc = hip_fail(e, "dpc_server_create");
+    dpc_server_destroy(srv.release());
+    return rc;
+  }
+  dpc_server* s = srv.release();
+  s->worker = std::thread([s] { s->run(); });
+  *out = s;
+  return DPC_OK;
+}
+
+int dpc_server_destroy(dpc_server* s)
+{
+  if (!s) return DPC_OK;
+  if (s->worker.joinable()) {
+    {
+      std::lock_guard<std::mutex> g(s->mu);
+      s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->worker.join();
+  }
+  for (auto& sl : s->slots) {
+    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+    if (sl.d_arena) (void)hipFree(sl.d_arena);
+    if (sl.h_arena) (void)hipHostFree(sl.h_arena);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  for (auto& sc : s->scenes) {
+    if (!sc.live) continue;
+    (void)hipFree(sc.box_a);
+    (void)hipFree(sc.box_b);
+    (void)hipFree(sc.planes);
+    if (sc.xform) (void)hipFree(sc.xform);
+  }
+  if (s->d_scene_table) (void)hipFree(s->d_scene_table);
+  delete s;
+  return DPC_OK;
+}
+
+int dpc_server_add_scene(dpc_server* s, const float* lower_upper, uint64_t n, const float* matrices, uint64_t n_mat,
+                         const uint32_t* xform, uint32_t* scene_id)
+{
+  if (!s || !scene_id || (n && !lower_upper) || (n_mat && !matrices) || n == 0 || n_mat == 0)
+    return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_add_scene: bad argument");
+  if (!xform && n_mat < n) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_add_scene: n_matrices < n_objects");
+  const uint64_t bytes = n * 24 + n_mat * 64 + (xform ? n * 4 : 0);
+  uint32_t id;
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    if (s->scene_bytes + bytes + s->arena_total > s->cfg.hbm_budget_bytes)
+      return fail(DPC_ERR_OVER_BUDGET, "dpc_server_add_scene: the scene does not fit the budget");
+    id = 0;
+    while (id < s->scenes.size() && s->scenes[id].live) ++id;
+    if (id == s->scenes.size()) return fail(DPC_ERR_OVER_BUDGET, "dpc_server_add_scene: all scene slots are in use");
+    s->scenes[id].live = true;  // reserve the slot
+    s->scenes[id].busy = true;  // not schedulable until uploaded
+    s->scene_bytes += bytes;
+  }
+  Scene sc;
+  sc.n = n;
+  sc.n_mat = n_mat;
+  sc.bytes = bytes;
+  sc.live = true;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc.box_a), n * 16);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc.box_b), n * 8);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc.planes), n_mat * 64);
+  if (e == hipSuccess && xform) e = hipMalloc(reinterpret_cast<void**>(&sc.xform), n * 4);
+  void *d_lu = nullptr, *d_aos = nullptr;
+  if (e == hipSuccess) e = hipMalloc(&d_lu, n * 24);
+  if (e == hipSuccess) e = hipMalloc(&d_aos, n_mat * 64);
+  if (e == hipSuccess) e = hipMemcpy(d_lu, lower_upper, n * 24, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d_aos, matrices, n_mat * 64, hipMemcpyHostToDevice);
+  if (e == hipSuccess && xform) e = hipMemcpy(sc.xform, xform, n * 4, hipMemcpyHostToDevice);
+  int rc = e == hipSuccess ? DPC_OK : hip_fail(e, "dpc_server_add_scene: upload");
+  if (rc == DPC_OK) rc = dpc_boxes_from_lower_upper(static_cast<const float*>(d_lu), n, sc.box_a, sc.box_b, nullptr);
+  if (rc == DPC_OK) rc = dpc_matrices_from_aos(d_aos, 64, n_mat, sc.planes, n_mat, nullptr);
+  dpc_scene_desc desc{sc.box_a, sc.box_b, sc.xform, sc.planes, n_mat, n};
+  if (rc == DPC_OK) {
+    e = hipMemcpy(s->d_scene_table + id, &desc, sizeof desc, hipMemcpyHostToDevice);
+    if (e != hipSuccess) rc = hip_fail(e, "scene table");
+  }
+  if (rc == DPC_OK) {
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = hip_fail(e, "dpc_server_add_scene");
+  }
+  if (d_lu) (void)hipFree(d_lu);
+  if (d_aos) (void)hipFree(d_aos);
+  std::lock_guard<std::mutex> g(s->mu);
+  if (rc != DPC_OK) {
+    if (sc.box_a) (void)hipFree(sc.box_a);
+    if (sc.box_b) (void)hipFree(sc.box_b);
+    if (sc.planes) (void)hipFree(sc.planes);
+    if (sc.xform) (void)hipFree(sc.xform);
+    s->scenes[id] = Scene{};
+    s->scene_bytes -= bytes;
+    return rc;
+  }
+  s->scenes[id] = sc;
+  s->stats.scene_bytes = s->scene_bytes;
+  *scene_id = id;
+  return DPC_OK;
+}
+
+int dpc_server_remove_scene(dpc_server* s, uint32_t id)
+{
+  if (!s) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_remove_scene: null server");
+  Scene sc;
+  {
+    std::unique_lock<std::mutex> lock(s->mu);
+    if (id >= s->scenes.size() || !s->scenes[id].live) return fail(DPC_ERR_NOT_FOUND, "no such scene");
+    if (s->scenes[id].busy || s->scenes[id].pending)
+      return fail(DPC_ERR_STATE, "dpc_server_remove_scene: the scene has requests in flight");
+    sc = s->scenes[id];
+    s->scenes[id] = Scene{};
+    s->scene_bytes -= sc.bytes;
+    s->stats.scene_bytes = s->scene_bytes;
+  }
+  dpc_scene_desc zero{};
+  (void)hipMemcpy(s->d_scene_table + id, &zero, sizeof zero, hipMemcpyHostToDevice);
+  (void)hipFree(sc.box_a);
+  (void)hipFree(sc.box_b);
+  (void)hipFree(sc.planes);
+  if (sc.xform) (void)hipFree(sc.xform);
+  return DPC_OK;
+}
+
+int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32_t n_frusta, const uint32_t* didx,
+                      const float* dmats, uint32_t n_delta, uint64_t* masks_out, uint64_t mask_capacity_words,
+                      uint64_t* request_id)
+{
+  if (!s || !request_id || !vps || !masks_out || n_frusta == 0 || n_frusta > kMaxFrusta ||
+      (n_delta && (!didx || !dmats)))
+    return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: bad argument");
+  auto r = std::make_shared<Request>();
+  r->scene = scene_id;
+  r->vps = vps;
+  r->n_frusta = n_frusta;
+  r->didx = didx;
+  r->dmats = dmats;
+  r->n_delta = n_delta;
+  r->masks_out = masks_out;
+  std::lock_guard<std::mutex> g(s->mu);
+  if (scene_id >= s->scenes.size() || !s->scenes[scene_id].live) return fail(DPC_ERR_NOT_FOUND, "no such scene");
+  const Scene& sc = s->scenes[scene_id];
+  // delta indices are checked here, on the host, so the scatter never writes outside the scene
+  for (uint32_t k = 0; k < n_delta; ++k)
+    if (didx[k] >= sc.n_mat) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: delta index out of range");
+  r->words = (sc.n + 63) / 64;
+  if (mask_capacity_words < r->words * n_frusta)
+    return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: the mask buffer is smaller than n_frusta x ceil(n/64) words");
+  r->dev_bytes = align_up(4ull * n_delta, 256) + align_up(64ull * n_delta, 256) + sizeof(dpc::ServeEntry) +
+                 (sizeof(dpc::KPair) + 64) * n_frusta + 8 * r->words * n_frusta + 1024;
+  if (r->dev_bytes > s->slots[0].cap)
+    return fail(DPC_ERR_OVER_BUDGET, "dpc_server_submit: the request alone exceeds the in-flight budget");
+  r->id = s->next_id++;
+  r->t_submit = s->now();
+  s->scenes[scene_id].pending++;
+  s->requests[r->id] = r;
+  s->pending.push_back(r);
+  *request_id = r->id;
+  s->cv_work.notify_one();
+  return DPC_OK;
+}
+
+int dpc_server_wait(dpc_server* s, uint64_t request_id, double timeout_ms, dpc_request_info* info)
+{
+  if (!s) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_wait: null server");
+  std::unique_lock<std::mutex> lock(s->mu);
+  auto it = s->requests.find(request_id);
+  if (it == s->requests.end()) return fail(DPC_ERR_NOT_FOUND, "dpc_server_wait: unknown request");
+  std::shared_ptr<Request> r = it->second;
+  auto ready = [&] { return r->done; };
+  if (timeout_ms < 0)
+    s->cv_done.wait(lock, ready);
+  else if (!s->cv_done.wait_for(lock, std::chrono::duration<double, std::milli>(timeout_ms), ready))
+    return fail(DPC_ERR_STATE, "dpc_server_wait: timed out");
+  if (info) {
+    info->submit_us = r->t_submit;
+    info->admit_us = r->t_admit;
+    info->done_us = r->t_done;
+    info->batch = r->batch;
+    info->batch_requests = r->batch_requests;
+    info->mask_words = r->words * r->n_frusta;
+    info->status = r->status;
+  }
+  s->requests.erase(it);
+  return r->status;
+}
+
+int dpc_server_stats_get(dpc_server* s, dpc_server_stats* st)
+{
+  if (!s || !st) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_stats_get: null argument");
+  std::lock_guard<std::mutex> g(s->mu);
+  *st = s->stats;
+  return DPC_OK;
+}
+
+double dpc_server_now_us(dpc_server* s) { return s ? s->now() : 0.0; }
+
+}  // extern "C"

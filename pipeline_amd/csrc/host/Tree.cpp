@@ -329,6 +329,7 @@ void Tree::compute(Mat44f const&)
   check(dpc_memcpy_d2h(m_lastEvent.data(), d.bits.p, m_lastEvent.size() * 8, d.stream), "download event");
   check(dpc_fill_u32(reinterpret_cast<uint32_t*>(d.dirty_world.p), 0u, (cap + 4) / 4, d.stream), "clear dirty");
   check(dpc_stream_sync(d.stream), "compute sync");
+  ++m_computeCount;
   for (TreeObserver* o : m_observers) o->onWorldMatricesChanged(m_lastEvent, cap);
   std::fill(m_dirtyLocal.begin(), m_dirtyLocal.end(), 0);
   m_anyDirty = false;

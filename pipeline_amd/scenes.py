@@ -358,3 +358,68 @@ def batched_scene(seed: int = 3, n: int = 2_097_152, n_groups: int = 64, n_casca
     words = (sizes[pg] + 63) // 64
     off = np.concatenate([[0], np.cumsum(words)[:-1]]).astype(np.uint64)
     return BatchedScene(lower, upper, mats, first, sizes.astype(np.uint64), vps, kinds, pf, pg, off)
+
+
+# ---------------------------------------------------------------------------
+# serving: resident scenes and a request stream
+# ---------------------------------------------------------------------------
+@dataclass
+class ServingScene:
+    lower: np.ndarray      # (n, 3)
+    upper: np.ndarray      # (n, 3)
+    matrices: np.ndarray   # (n, 4, 4), object i uses matrix i
+    perm: np.ndarray       # (n,) u32, a permutation: delta index sets are slices of it (distinct indices)
+    pool: np.ndarray       # (ceil(n/10), 4, 4) replacement matrices: delta matrices are slices of it
+
+    @property
+    def n(self) -> int:
+        return self.lower.shape[0]
+
+
+@dataclass
+class ServingRequest:
+    scene: int
+    vps: np.ndarray        # (F, 4, 4) f32
+    delta_idx: np.ndarray  # (k,) u32 view into the scene's perm
+    delta_mats: np.ndarray  # (k, 4, 4) f32 view into the scene's pool
+    arrival_us: float      # offset from the start of the stream
+
+
+def serving_scenes(seed: int = 4, n_scenes: int = 16, size_range=(10_000, 2_000_000),
+                   extent: float = 1000.0) -> list:
+    """Scenes of log-uniform size with the flat 1M-object distributions (box
+    centres in [-extent, extent]^3, rotation x scale [0.5,2] x translation [-5,5]^3)."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(size_range[0]), np.log(size_range[1]), n_scenes)).astype(np.int64)
+    scenes = []
+    for n in sizes:
+        n = int(n)
+        lower, upper = random_boxes(rng, n, (-extent, extent))
+        mats = trs_matrices(rng, n)
+        perm = rng.permutation(n).astype(np.uint32)
+        pool = trs_matrices(rng, (n + 9) // 10)
+        scenes.append(ServingScene(lower, upper, mats, perm, pool))
+    return scenes
+
+
+def serving_requests(scenes: list, seed: int, n_requests: int, rate_per_s: float, frusta=(1, 32),
+                     delta_fraction=(0.01, 0.10), extent: float = 1000.0) -> list:
+    """Poisson arrivals at `rate_per_s`; each request names a scene, carries 1-32
+    random perspective frusta (fov 45, far 2000, eye in [-50,50]^3, random
+    direction) and a matrix delta covering 1-10% of the scene's objects."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1e6 / rate_per_s, n_requests) if rate_per_s > 0 else np.zeros(n_requests)
+    t = np.cumsum(gaps) - gaps[0]
+    out = []
+    for i in range(n_requests):
+        s = int(rng.integers(0, len(scenes)))
+        sc = scenes[s]
+        f = int(rng.integers(frusta[0], frusta[1] + 1))
+        vps = np.stack([perspective_vp(rng.uniform(-50, 50, 3), rng.normal(size=3), 45.0, 16 / 9, 0.1, 2 * extent)
+                        for _ in range(f)]).astype(np.float32)
+        k = max(1, int(round(rng.uniform(*delta_fraction) * sc.n)))
+        k = min(k, sc.pool.shape[0], sc.n)
+        o1 = int(rng.integers(0, sc.n - k + 1))
+        o2 = int(rng.integers(0, sc.pool.shape[0] - k + 1))
+        out.append(ServingRequest(s, vps, sc.perm[o1:o1 + k], sc.pool[o2:o2 + k], float(t[i])))
+    return out

@@ -1,0 +1,193 @@
+"""GPU tests of the serving loop (dpc_server_*): resident scenes, requests that
+apply a matrix delta and then cull the scene against 1-32 frusta, batched across
+scenes and overlapped on several streams.
+
+Parity: every request's masks equal the oracle's cull of its scene with the
+deltas of that scene's earlier requests applied (host replay in submission
+order), bit for bit.  The same request stream run one request at a time per
+scene (alone) gives the same masks as the mixed, batched run."""
+import numpy as np
+import pytest
+
+from gpu_util import mismatches
+from pipeline_amd import _lib
+from pipeline_amd import scenes as S
+from pipeline_amd.serving import PinnedArray, Server
+
+pytestmark = pytest.mark.gpu
+
+
+class Replay:
+    """Host replay of the scenes' matrices in request order, checked by the oracle."""
+
+    def __init__(self, orc, scenes):
+        self.orc = orc
+        self.scenes = scenes
+        self.mats = [sc.matrices.reshape(-1, 16).copy() for sc in scenes]
+        self.boxes = [S.extents(sc.lower, sc.upper) for sc in scenes]
+
+    def apply_and_cull(self, req) -> np.ndarray:
+        s = req.scene
+        self.mats[s][req.delta_idx] = req.delta_mats.reshape(-1, 16)
+        l4, e4 = self.boxes[s]
+        n = self.scenes[s].n
+        obbs = self.orc.obb_build_group(l4, e4, np.arange(n), self.mats[s])
+        return np.concatenate([self.orc.cull_obbs(obbs, vp) for vp in req.vps])
+
+
+def _run_mix(scenes, reqs, streams=3, max_batch=3):
+    out = []
+    with Server(max_scenes=16, max_batch_requests=max_batch, streams=streams, hbm_budget_bytes=4 << 30) as srv:
+        ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scenes]
+        pending = []
+        for r in reqs:
+            m = np.zeros(srv.mask_words(ids[r.scene], r.vps.shape[0]), np.uint64)
+            pending.append((srv.submit(ids[r.scene], r.vps, m, r.delta_idx, r.delta_mats), m))
+        for rid, m in pending:
+            out.append((srv.wait(rid, 60_000.0), m))
+        stats = srv.stats()
+    return out, stats
+
+
+def _run_alone(scene, reqs):
+    out = []
+    with Server(max_scenes=1, max_batch_requests=1, streams=1, hbm_budget_bytes=1 << 30) as srv:
+        sid = srv.add_scene(scene.lower, scene.upper, scene.matrices)
+        for r in reqs:
+            m = np.zeros(srv.mask_words(sid, r.vps.shape[0]), np.uint64)
+            rid = srv.submit(sid, r.vps, m, r.delta_idx, r.delta_mats)
+            info = srv.wait(rid, 60_000.0)
+            assert info.status == 0 and info.batch_requests == 1
+            out.append(m)
+    return out
+
+
+def test_serve_mix_equals_alone_and_oracle(orc):
+    scenes = S.serving_scenes(seed=11, n_scenes=6, size_range=(3_000, 120_000))
+    reqs = S.serving_requests(scenes, seed=12, n_requests=48, rate_per_s=0.0)
+    mix, stats = _run_mix(scenes, reqs)
+    assert all(info.status == 0 for info, _ in mix)
+    assert stats["requests_done"] == len(reqs)
+    assert stats["tests"] == sum(scenes[r.scene].n * r.vps.shape[0] for r in reqs)
+    assert stats["delta_matrices"] == sum(r.delta_idx.size for r in reqs)
+    # batching across scenes happened, and batches of different streams overlapped
+    assert max(info.batch_requests for info, _ in mix) > 1
+    spans = sorted({(info.batch, info.admit_us, info.done_us) for info, _ in mix})
+    assert any(b[1] < a[2] and a[0] != b[0] for a, b in zip(spans, spans[1:])), "no two batches were in flight together"
+
+    # the oracle, replaying every scene's deltas in submission order
+    rep = Replay(orc, scenes)
+    bad = 0
+    for r, (info, m) in zip(reqs, mix):
+        want = rep.apply_and_cull(r)
+        n = scenes[r.scene].n
+        words = (n + 63) // 64
+        for f in range(r.vps.shape[0]):
+            bad += mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
+    assert bad == 0, f"{bad} visibility bits differ from the oracle"
+
+    # alone == mix, per scene
+    for s, sc in enumerate(scenes):
+        mine = [i for i, r in enumerate(reqs) if r.scene == s]
+        alone = _run_alone(sc, [reqs[i] for i in mine])
+        for i, m in zip(mine, alone):
+            assert np.array_equal(m, mix[i][1]), f"scene {s} request {i}: alone != mix"
+
+
+def test_serve_budget_admission(orc):
+    """Requests are admitted while the in-flight arena of a stream has room: two
+    whole-scene deltas never share a batch, small requests do."""
+    scenes = S.serving_scenes(seed=21, n_scenes=4, size_range=(50_000, 50_001))
+    budget = 64 << 20
+    streams = 4
+    per_stream = ((budget // 4) // streams + 4095) // 4096 * 4096
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(16):
+        s = i % 4
+        sc = scenes[s]
+        big = i % 3 == 0
+        f = 64 if big else 32
+        vps = np.stack([S.perspective_vp(rng.uniform(-50, 50, 3), rng.normal(size=3), 45.0, 16 / 9, 0.1, 2000.0)
+                        for _ in range(f)]).astype(np.float32)
+        if big:  # whole-scene delta: about 3.8 MB of a 4 MiB arena
+            reqs.append(S.ServingRequest(s, vps, sc.perm, S.trs_matrices(rng, sc.n), 0.0))
+        else:
+            reqs.append(S.ServingRequest(s, vps, sc.perm[:1000], sc.pool[:1000], 0.0))
+    with Server(max_scenes=4, max_batch_requests=8, streams=streams, hbm_budget_bytes=budget) as srv:
+        ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scenes]
+        pend = []
+        for r in reqs:
+            m = np.zeros(srv.mask_words(ids[r.scene], r.vps.shape[0]), np.uint64)
+            pend.append((r, srv.submit(ids[r.scene], r.vps, m, r.delta_idx, r.delta_mats), m))
+        done = [(r, srv.wait(rid, 60_000.0), m) for r, rid, m in pend]
+        st = srv.stats()
+    assert all(info.status == 0 for _, info, _ in done)
+    assert 0 < st["peak_inflight_bytes"] <= per_stream
+    bigs_per_batch = {}
+    for r, info, _ in done:
+        if r.delta_idx.size == scenes[r.scene].n:
+            bigs_per_batch[info.batch] = bigs_per_batch.get(info.batch, 0) + 1
+    assert bigs_per_batch and max(bigs_per_batch.values()) == 1
+    assert max(info.batch_requests for _, info, _ in done) > 1
+    rep = Replay(orc, scenes)
+    n = scenes[0].n
+    words = (n + 63) // 64
+    for r, _, m in done:
+        want = rep.apply_and_cull(r)
+        bad = sum(mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
+                  for f in range(r.vps.shape[0]))
+        assert bad == 0
+
+
+def test_serve_errors_and_scene_slots(orc):
+    scenes = S.serving_scenes(seed=31, n_scenes=2, size_range=(20_000, 20_001))
+    sc = scenes[0]
+    vp = S.perspective_vp(np.zeros(3), np.array([0.0, 0.0, -1.0]), 45.0, 16 / 9, 0.1, 2000.0)[None]
+    with Server(max_scenes=2, max_batch_requests=4, streams=1, hbm_budget_bytes=64 << 20) as srv:
+        # a scene larger than the budget
+        big = S.serving_scenes(seed=32, n_scenes=1, size_range=(600_000, 600_001))[0]
+        with pytest.raises(_lib.DpcError) as e:
+            srv.add_scene(big.lower, big.upper, big.matrices)
+        assert e.value.code == _lib.DPC_ERR_OVER_BUDGET
+        a = srv.add_scene(sc.lower, sc.upper, sc.matrices)
+        b = srv.add_scene(scenes[1].lower, scenes[1].upper, scenes[1].matrices)
+        with pytest.raises(_lib.DpcError) as e:  # both slots are taken
+            srv.add_scene(sc.lower, sc.upper, sc.matrices)
+        assert e.value.code == _lib.DPC_ERR_OVER_BUDGET
+        m = np.zeros(srv.mask_words(a, 1), np.uint64)
+        with pytest.raises(_lib.DpcError) as e:  # unknown scene
+            srv.submit(7, vp, m)
+        assert e.value.code == _lib.DPC_ERR_NOT_FOUND
+        with pytest.raises(_lib.DpcError) as e:  # delta index out of range
+            srv.submit(a, vp, m, np.array([sc.n], np.uint32), sc.pool[:1])
+        assert e.value.code == _lib.DPC_ERR_INVALID_ARGUMENT
+        with pytest.raises(_lib.DpcError) as e:  # mask buffer too small
+            srv.submit(a, vp, m[:-1])
+        assert e.value.code == _lib.DPC_ERR_INVALID_ARGUMENT
+        with pytest.raises(_lib.DpcError) as e:  # 65 frusta
+            srv.submit(a, np.repeat(vp, 65, axis=0), np.zeros(65 * m.size, np.uint64))
+        assert e.value.code == _lib.DPC_ERR_INVALID_ARGUMENT
+        with pytest.raises(_lib.DpcError) as e:  # a request larger than the in-flight arena (16 MiB)
+            k = 250_000
+            srv.submit(a, vp, m, (np.arange(k) % sc.n).astype(np.uint32), np.zeros((k, 16), np.float32))
+        assert e.value.code == _lib.DPC_ERR_OVER_BUDGET
+        # remove a scene, add another into its slot, and cull it
+        srv.remove_scene(b)
+        with pytest.raises(_lib.DpcError) as e:
+            srv.remove_scene(b)
+        assert e.value.code == _lib.DPC_ERR_NOT_FOUND
+        c = srv.add_scene(scenes[1].lower, scenes[1].upper, scenes[1].matrices)
+        assert c == b
+        pinned = PinnedArray(srv.mask_words(c, 1), np.uint64)  # the DMA path
+        m2 = pinned.array
+        info = srv.wait(srv.submit(c, vp, m2), 60_000.0)
+        assert info.status == 0
+        m2 = m2.copy()
+        pinned.free()
+        l4, e4 = S.extents(scenes[1].lower, scenes[1].upper)
+        want = orc.cull_group(l4, e4, np.arange(scenes[1].n), scenes[1].matrices, vp[0])
+        assert mismatches(m2, want, scenes[1].n) == 0
+        with pytest.raises(_lib.DpcError) as e:  # waited for already
+            srv.wait(info.request_id, 10.0)
+        assert e.value.code == _lib.DPC_ERR_NOT_FOUND
