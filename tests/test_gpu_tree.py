@@ -69,9 +69,14 @@ def oracle_world(orc, ts):
     return world.reshape(-1, 4, 4), event
 
 
-@pytest.mark.parametrize("n_nodes,seed", [(37, 1), (2_000, 5), (50_000, 8)])
-def test_propagate_matches_oracle(orc, n_nodes, seed):
-    ts = scenes.tree_scene(n_nodes, seed)
+@pytest.mark.parametrize("n_nodes,seed,kw", [
+    (37, 1, {}), (2_000, 5, {}), (50_000, 8, {}),
+    # two levels above the single-workgroup size, then a small one whose parents
+    # were written by an earlier launch
+    (6_500, 9, {"roots": 3_000, "p": 0.99}),
+])
+def test_propagate_matches_oracle(orc, n_nodes, seed, kw):
+    ts = scenes.tree_scene(n_nodes, seed, **kw)
     t = DeviceTree(ts)
     t.propagate()
     got = t.world_by_slot()
