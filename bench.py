@@ -52,6 +52,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--skip-extra", action="store_true", help="only the batched and per-view lines")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="launch only a 512 MiB device copy, the batched kernel and the flat kernel a few times "
+                         "each (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes); prints no JSON")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of each serving-rate run")
     ap.add_argument("--tune", default="", help="batch config k,block,waves_per_cu,max_frusta")
     return ap.parse_args()
@@ -104,6 +107,8 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     of the sampled scenes in submission order)."""
     import threading
 
+    from pipeline_amd import _lib as _lib_mod
+    from pipeline_amd import device as dev_mod
     from pipeline_amd import scenes
     from pipeline_amd.serving import PinnedArray, Server
 
@@ -121,6 +126,24 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
         pm.array[:] = sc.pool
         pinned += [pp, pm]
         sc.perm, sc.pool = pp.array, pm.array
+    # PCIe copy rates between pinned host memory and the device (what every
+    # request's delta and masks cross)
+    pc_bytes = 256 << 20
+    ph = PinnedArray(pc_bytes // 4, np.uint32)
+    pd = dev_mod.DeviceArray(pc_bytes // 4, np.uint32)
+    pcie = {}
+    for name, fn in (("h2d", lambda: lib.dpc_memcpy_h2d(pd.ptr, ph.array.ctypes.data, pc_bytes, None)),
+                     ("d2h", lambda: lib.dpc_memcpy_d2h(ph.array.ctypes.data, pd.ptr, pc_bytes, None))):
+        fn()
+        lib.dpc_device_sync()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            _lib_mod.check(fn())
+        lib.dpc_device_sync()
+        pcie[name + "_gbs"] = 3 * pc_bytes / (time.perf_counter() - t0) / 1e9
+    pd.free()
+    ph.free()
+    log(f"serving: pinned copies {pcie['h2d_gbs']:.1f} GB/s host->device, {pcie['d2h_gbs']:.1f} GB/s device->host")
     n_slots, slot_words = 192, 32 * ((max(sizes) + 63) // 64)
     out_pool = PinnedArray((n_slots, slot_words), np.uint64)
     srv = Server(max_scenes=16, max_batch_requests=16, streams=4, hbm_budget_bytes=32 << 30)
@@ -185,7 +208,14 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
         t_last = max(x.done_us for x in results)
         tests = sum(sizes[r.scene] * r.vps.shape[0] for r in reqs)
         batches = len({x.batch for x in results})
-        runs.append(dict(offered_rate=rate, requests=n_req, achieved_submit_rate=n_req / ((t_submitted - start) * 1e-6),
+        st_now = srv.stats()
+        issue_us = st_now["issue_us"] - (runs[-1]["_issue_total"] if runs else 0.0)
+        delta_bytes = sum(68 * r.delta_idx.size for r in reqs)
+        mask_bytes = sum(8 * ((sizes[r.scene] + 63) // 64) * r.vps.shape[0] for r in reqs)
+        span_s = (t_last - t_first) * 1e-6
+        runs.append(dict(_issue_total=st_now["issue_us"], scheduler_issue_frac=issue_us * 1e-6 / span_s,
+                         pcie_h2d_gbs=delta_bytes / span_s / 1e9, pcie_d2h_gbs=mask_bytes / span_s / 1e9,
+                         offered_rate=rate, requests=n_req, achieved_submit_rate=n_req / ((t_submitted - start) * 1e-6),
                          p50_us=float(np.percentile(lat, 50)), p99_us=float(np.percentile(lat, 99)),
                          mean_us=float(lat.mean()), tests=tests,
                          tests_per_s=tests / ((t_last - t_first) * 1e-6), batches=batches,
@@ -193,7 +223,8 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
                          delta_matrices=int(sum(r.delta_idx.size for r in reqs))))
         log(f"serving @ {rate}/s: p50 {runs[-1]['p50_us']:.0f} us, p99 {runs[-1]['p99_us']:.0f} us, "
             f"{runs[-1]['tests_per_s'] / 1e9:.1f} G tests/s, {batches} batches, "
-            f"submit rate {runs[-1]['achieved_submit_rate']:.0f}/s")
+            f"submit rate {runs[-1]['achieved_submit_rate']:.0f}/s, PCIe in {runs[-1]['pcie_h2d_gbs']:.1f} GB/s, "
+            f"out {runs[-1]['pcie_d2h_gbs']:.1f} GB/s, scheduler issuing {runs[-1]['scheduler_issue_frac']:.0%}")
         if ri == 0:
             # oracle replay of the sampled scenes' deltas in submission order
             mism, checked_tests = 0, 0
@@ -215,18 +246,80 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
             check = dict(sampled_requests=len(sample), checked_tests=checked_tests, mismatches=mism)
             log(f"serving oracle check: {check}")
     sys.setswitchinterval(switch)
+    for r in runs:
+        r.pop("_issue_total", None)
     st = srv.stats()
     srv.close()
     out_pool.free()
     for p in pinned:
         p.free()
-    return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check,
+    return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check, pinned_copy=pcie,
                 server={"streams": 4, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
                         "scene_bytes": st["scene_bytes"], "peak_inflight_bytes": st["peak_inflight_bytes"]})
 
 
+PMC_COPY_BYTES = 512 << 20
+
+
+def measured_traffic(kernel: str):
+    """HBM-side bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE passes of `bench.py --pmc-probe`, each calibrated on
+    a 512 MiB copy), or None when the summary is absent."""
+    f = ROOT / "profiles" / "cull_batch_kernel_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            return int(line.split(":")[1])
+    return None
+
+
+def pmc_probe():
+    """Kernels for the counter passes: a plain device copy of PMC_COPY_BYTES
+    (calibrates FETCH_SIZE / WRITE_SIZE to bytes), then the batched launch of
+    the headline workload and the flat single-frustum launch, 5 times each."""
+    from pipeline_amd import _lib, scenes
+    from pipeline_amd import device as dev
+    from pipeline_amd import autotune
+    lib = _lib.load()
+    stream = dev.Stream()
+    src = dev.DeviceArray(PMC_COPY_BYTES // 4, np.uint32)
+    dst = dev.DeviceArray(PMC_COPY_BYTES // 4, np.uint32)
+    src.fill_u32(1, stream.handle)
+    for _ in range(3):
+        _lib.check(lib.dpc_memcpy_d2d(dst.ptr, src.ptr, PMC_COPY_BYTES, stream.handle))
+    stream.sync()
+    src.free()
+    dst.free()
+    bs = scenes.batched_scene()
+    scene = dev.upload_scene(bs.lower, bs.upper, bs.matrices)
+    plan = dev.batched_plan(bs, autotune.best_batch_config(bs.n, bs.tests, bs.vps.shape[0]))
+    plan.bind([scene])
+    d_vps = dev.DeviceArray.from_host(bs.vps.reshape(-1, 16))
+    masks = dev.DeviceArray(bs.mask_words, np.uint64)
+    for _ in range(5):
+        plan.cull(d_vps, masks, stream.handle)
+    stream.sync()
+    scene.free()
+    plan.destroy()
+    fs = scenes.config1_scene()
+    copies = [dev.upload_scene(fs.lower, fs.upper, fs.matrices) for _ in range(4)]
+    fmask = dev.DeviceArray(dev.mask_words(fs.n), np.uint64)
+    vp16 = dev.vp_arg(fs.vp)
+    for k in range(5):
+        s = copies[k % 4]
+        _lib.check(lib.dpc_cull(s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr, s.ld, vp16, fmask.ptr,
+                                stream.handle))
+    stream.sync()
+    stream.destroy()
+    log("pmc probe: 3 copies of 512 MiB, 5 batched launches, 5 flat launches")
+
+
 def main():
     args = parse()
+    if args.pmc_probe:
+        pmc_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}); "
               "see the sharding design in DESIGN.md", file=sys.stderr)
@@ -400,6 +493,29 @@ def main():
         wall_t, k_mean_t, _ = run_timed(dev, lib, stream, max(args.steps // 4, 5), args.warmup,
                                         lambda: tgraph.launch(stream))
         tgraph.destroy()
+        # parity of this run's results: world matrices vs the FP32 oracle (ulp) and
+        # an FP64 evaluation of the same recurrence; the cull mask vs the oracle
+        got_w = np.transpose(d_world.download(stream.handle), (1, 0, 2))
+        ref_off, ref_entries = ts.reference_levels()
+        loc = np.ascontiguousarray(ts.local_by_index().reshape(-1, 16))
+        w32 = np.zeros_like(loc)
+        w32[0] = np.eye(4, dtype=np.float32).reshape(16)
+        orc.tree_compute(ref_off, ref_entries, loc, w32, np.ones(n_slots, np.uint8), np.zeros(n_slots, np.uint8))
+        want_w = w32.reshape(-1, 4, 4)[ts.index_of_slot]
+        ia = got_w[1:].view(np.int32).astype(np.int64)
+        ib = want_w[1:].view(np.int32).astype(np.int64)
+        ia = np.where(ia < 0, -(ia & 0x7FFFFFFF), ia)
+        ib = np.where(ib < 0, -(ib & 0x7FFFFFFF), ib)
+        max_ulp = int(np.abs(ia - ib).max())
+        w64 = orc.tree_compute_f64(ref_off, ref_entries, ts.local_by_index().astype(np.float64))[ts.index_of_slot]
+        scale = np.abs(w64[1:]).max(axis=(1, 2), keepdims=True)
+        max_rel = float((np.abs(got_w[1:].astype(np.float64) - w64[1:]) / scale).max())
+        l4t, e4t = scenes.extents(ts.obj_lower, ts.obj_upper)
+        want_t = orc.cull_group(l4t, e4t, ts.obj_slot, want_w, ts.vp)
+        got_t = tmask.download(stream.handle)
+        mism_t = int(np.unpackbits((got_t ^ want_t).view(np.uint8)).sum())
+        log(f"tree parity: max ulp vs FP32 oracle {max_ulp}, max relative error vs FP64 {max_rel:.2e}, "
+            f"cull mismatches {mism_t}")
         n_nodes = n_slots - 1
         m_obj = objs.n_objects
         prop_bytes = n_nodes * (64 + 64 + 4) + 64 * (n_slots - int(np.diff(ts.level_offsets)[-1]))
@@ -407,7 +523,9 @@ def main():
         result["tree"] = dict(nodes=n_nodes, levels=len(ts.level_offsets) - 1, objects=m_obj,
                               us_per_step=wall_t / max(args.steps // 4, 5) * 1e6, step_event_us=k_mean_t * 1e3,
                               achieved_gbs=(prop_bytes + cull_bytes) / (k_mean_t * 1e-3) / 1e9,
-                              tests_per_s=m_obj / (wall_t / max(args.steps // 4, 5)))
+                              tests_per_s=m_obj / (wall_t / max(args.steps // 4, 5)),
+                              world_max_ulp_vs_oracle=max_ulp, world_max_rel_err_vs_fp64=max_rel,
+                              cull_mismatches=mism_t)
         log(f"tree: {result['tree']['us_per_step']:.1f} us/step (propagate + cull)")
         for d in (d_local, d_world, d_parent, d_dirty, tmask):
             d.free()
@@ -439,7 +557,9 @@ def main():
                        "oracle_sampled_tests": checked},
         "speedup_vs_per_view": value / per_view_value,
         "roofline": {"bound": "valu", "achieved": achieved_tops, "peak": peak_tops, "unit": "TFLOP/s",
-                     "frac": achieved_tops / peak_tops, "traffic": None,
+                     "frac": achieved_tops / peak_tops, "traffic": measured_traffic("cull_batch_kernel"),
+                     "traffic_unit": "bytes per launch beyond L2 (FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
+                                     "profiles/cull_batch_kernel_traffic.md)",
                      "kernel": "cull_batch_kernel", "kernel_us": k_mean_b * 1e3,
                      "basis": f"{OPS_PER_OBJECT} ops/object + {OPS_PER_TEST} ops/test (unfused FP32 add/mul and "
                               "compares); peak = 256 CU x 64 lanes x 2.4 GHz, no FMA, no packed FP32"},

@@ -114,6 +114,20 @@ class Tree {
   size_t deviceLd() const;
   const uint32_t* deviceSlotOfIndex() const;
   void* stream() const;
+  // World-matrix inverse transposes, dp::math::invertTranspose
+  // (dp/math/Matmnt.h:1171-1201), kept current by compute() for every world
+  // matrix it changed: the reference renderer computes them on the host, one
+  // changed index at a time (ShaderManagerTransformsRiXFx::updateTransforms,
+  // dp/sg/renderer/rix/gl/src/ShaderManagerRiXFx.cpp:162-182).  Off by default.
+  // A singular world matrix (det == 0) keeps its previous inverse transpose
+  // (zero for a new index); the reference leaves its output undefined there.
+  void setInverseTransposeEnabled(bool enabled);
+  bool inverseTransposeEnabled() const { return m_itEnabled; }
+  // Host copy by transform index (downloaded on first use after a compute).
+  Mat44f const* getWorldMatricesInverseTranspose() const;
+  // Row planes by slot, like deviceWorldPlanes().
+  const float* deviceWorldInverseTransposePlanes() const;
+
   // Number of compute() calls so far; consumers that cache something derived
   // from the world matrices compare it to know when to refresh.
   uint64_t computeCount() const { return m_computeCount; }
@@ -122,6 +136,7 @@ class Tree {
   struct Device;
   void ensureCapacity();
   void recompile();
+  void updateInverseTranspose();
 
   TreeLayout m_layout;
   TreeLayout::Compiled m_compiled;
@@ -131,6 +146,10 @@ class Tree {
   bool m_anyDirty = true;
   mutable std::vector<Mat44f> m_worldHost;
   mutable bool m_worldHostStale = true;
+  bool m_itEnabled = false;
+  bool m_itAll = true;  // the next compute() recomputes every inverse transpose
+  mutable std::vector<Mat44f> m_itHost;
+  mutable bool m_itHostStale = true;
   std::vector<uint64_t> m_lastEvent;
   std::vector<TreeObserver*> m_observers;
   std::unique_ptr<Device> m_dev;

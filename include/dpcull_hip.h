@@ -122,7 +122,7 @@ int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
              const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, void* stream);
 
 /* Launch shape of dpc_cull (process-wide; chosen by the autotuner).
- * waves_per_cu = 0: one wave per 64-object tile over a full grid;
+ * waves_per_cu = 0 (the default): one wave per 64-object tile over a full grid;
  * waves_per_cu > 0: a persistent grid of that many waves per CU, each wave
  * streaming over tiles with the next tile's loads in flight. */
 typedef struct dpc_cull_config {
@@ -209,6 +209,17 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
 int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of_slot, size_t n_slots,
                               uint32_t* d_bits, size_t n_bits, void* stream);
 
+/* World-matrix inverse transpose: dp::math::invertTranspose
+ * (dp/math/Matmnt.h:1171-1201), which the renderer applies to every changed
+ * world matrix (ShaderManagerTransformsRiXFx::updateTransforms,
+ * dp/sg/renderer/rix/gl/src/ShaderManagerRiXFx.cpp:162-182).  For each slot
+ * s < n with d_dirty == NULL or d_dirty[s] != 0 (the tree's dirty-world bytes):
+ * when det != 0 the rows of inverse(W[s])^T go to d_out_planes (row planes,
+ * out_ld) and d_ok[s] = 1; when det == 0 d_out_planes is left as it was and
+ * d_ok[s] = 0 (the reference returns false).  d_ok may be NULL. */
+int dpc_inverse_transpose(const float* d_planes, size_t ld, const uint8_t* d_dirty, size_t n, float* d_out_planes,
+                          size_t out_ld, uint8_t* d_ok, void* stream);
+
 /* ---- result diff (ResultBitSet::updateChanged, dp/culling/src/ResultBitSet.cpp:61-108) ---
  * changed = old ^ new over n bits; the indices of changed bits are written to
  * d_changed_idx in ascending order (BitArray::traverseBits order) and their
@@ -267,6 +278,7 @@ typedef struct dpc_server_stats {
   uint64_t scene_bytes;      /* device bytes of resident scenes */
   uint64_t peak_inflight_bytes;
   double busy_us;            /* sum over batches of launch-to-completion time */
+  double issue_us;           /* scheduler-thread time spent issuing batches (copies, launches) */
 } dpc_server_stats;
 
 int dpc_server_create(const dpc_server_config* cfg, dpc_server** server);

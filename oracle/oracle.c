@@ -247,3 +247,52 @@ size_t orc_result_diff(size_t n, const uint32_t* old_mask, const uint32_t* new_m
   }
   return count;
 }
+
+/* dp/math/Matmnt.h:1171-1201.  m[4*r + c] = M[r][c].  Every expression is
+ * evaluated left to right as in the reference: a - b + c = (a - b) + c, and
+ * -a * b = (-a) * b (the sign flip is exact). */
+int orc_invert_transpose(const float m[16], float out[16])
+{
+#define M(r, c) m[4 * (r) + (c)]
+  const float s0 = M(0, 0) * M(1, 1) - M(0, 1) * M(1, 0);
+  const float c5 = M(2, 2) * M(3, 3) - M(2, 3) * M(3, 2);
+  const float s1 = M(0, 0) * M(1, 2) - M(0, 2) * M(1, 0);
+  const float c4 = M(2, 1) * M(3, 3) - M(2, 3) * M(3, 1);
+  const float s2 = M(0, 0) * M(1, 3) - M(0, 3) * M(1, 0);
+  const float c3 = M(2, 1) * M(3, 2) - M(2, 2) * M(3, 1);
+  const float s3 = M(0, 1) * M(1, 2) - M(0, 2) * M(1, 1);
+  const float c2 = M(2, 0) * M(3, 3) - M(2, 3) * M(3, 0);
+  const float s4 = M(0, 1) * M(1, 3) - M(0, 3) * M(1, 1);
+  const float c1 = M(2, 0) * M(3, 2) - M(2, 2) * M(3, 0);
+  const float s5 = M(0, 2) * M(1, 3) - M(0, 3) * M(1, 2);
+  const float c0 = M(2, 0) * M(3, 1) - M(2, 1) * M(3, 0);
+  float det = s0 * c5;
+  det = det - s1 * c4;
+  det = det + s2 * c3;
+  det = det + s3 * c2;
+  det = det - s4 * c1;
+  det = det + s5 * c0;
+  if (!(det != 0.0f)) return 0;
+  const float inv = 1.0f / det;
+  float r[16];
+  /* r[4*i + j] = mOut[i][j] */
+  r[0 * 4 + 0] = ((M(1, 1) * c5 - M(1, 2) * c4) + M(1, 3) * c3) * inv;
+  r[1 * 4 + 0] = ((-M(0, 1) * c5 + M(0, 2) * c4) - M(0, 3) * c3) * inv;
+  r[2 * 4 + 0] = ((M(3, 1) * s5 - M(3, 2) * s4) + M(3, 3) * s3) * inv;
+  r[3 * 4 + 0] = ((-M(2, 1) * s5 + M(2, 2) * s4) - M(2, 3) * s3) * inv;
+  r[0 * 4 + 1] = ((-M(1, 0) * c5 + M(1, 2) * c2) - M(1, 3) * c1) * inv;
+  r[1 * 4 + 1] = ((M(0, 0) * c5 - M(0, 2) * c2) + M(0, 3) * c1) * inv;
+  r[2 * 4 + 1] = ((-M(3, 0) * s5 + M(3, 2) * s2) - M(3, 3) * s1) * inv;
+  r[3 * 4 + 1] = ((M(2, 0) * s5 - M(2, 2) * s2) + M(2, 3) * s1) * inv;
+  r[0 * 4 + 2] = ((M(1, 0) * c4 - M(1, 1) * c2) + M(1, 3) * c0) * inv;
+  r[1 * 4 + 2] = ((-M(0, 0) * c4 + M(0, 1) * c2) - M(0, 3) * c0) * inv;
+  r[2 * 4 + 2] = ((M(3, 0) * s4 - M(3, 1) * s2) + M(3, 3) * s0) * inv;
+  r[3 * 4 + 2] = ((-M(2, 0) * s4 + M(2, 1) * s2) - M(2, 3) * s0) * inv;
+  r[0 * 4 + 3] = ((-M(1, 0) * c3 + M(1, 1) * c1) - M(1, 2) * c0) * inv;
+  r[1 * 4 + 3] = ((M(0, 0) * c3 - M(0, 1) * c1) + M(0, 2) * c0) * inv;
+  r[2 * 4 + 3] = ((-M(3, 0) * s3 + M(3, 1) * s1) - M(3, 2) * s0) * inv;
+  r[3 * 4 + 3] = ((M(2, 0) * s3 - M(2, 1) * s1) + M(2, 2) * s0) * inv;
+#undef M
+  memcpy(out, r, sizeof r);
+  return 1;
+}

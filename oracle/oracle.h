@@ -43,6 +43,11 @@
  *                                           plane boundaries.
  *   orc_result_diff                         brute force (per-bit loop) and XOR
  *                                           invariants.
+ *   orc_invert_transpose                    bit-exact vs the compiled reference
+ *                                           dp::math::invertTranspose (oracle/_ref)
+ *                                           on committed golden vectors, plus
+ *                                           M^T x result = I in FP64 and the
+ *                                           singular (det == 0) cases.
  */
 #ifndef DPCULL_ORACLE_H
 #define DPCULL_ORACLE_H
@@ -118,6 +123,11 @@ void orc_tree_compute_f64(size_t n_levels, const uint64_t* level_offsets, const 
  * changed = new ^ old over n bits; writes the changed indices in ascending
  * order (BitArray::traverseBits order) and returns their count. */
 size_t orc_result_diff(size_t n, const uint32_t* old_mask, const uint32_t* new_mask, uint32_t* changed_idx);
+
+/* dp::math::invertTranspose (dp/math/Matmnt.h:1171-1201): the cofactor form of
+ * the inverse, written transposed.  Returns 1 and writes out when det != 0
+ * (NaN included); returns 0 and leaves out untouched when det == 0. */
+int orc_invert_transpose(const float m[16], float out[16]);
 
 #ifdef __cplusplus
 }

@@ -43,11 +43,13 @@ def load():
             "orc_tree_compute": [SZ, P, P, P, P, SZ, P, P, P],
             "orc_tree_compute_f64": [SZ, P, P, P, P],
             "orc_result_diff": [SZ, P, P, P],
+            "orc_invert_transpose": [P, P],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
             fn.argtypes = args
-            fn.restype = C.c_int if name == "orc_is_visible_obb" else (SZ if name == "orc_result_diff" else None)
+            fn.restype = C.c_int if name in ("orc_is_visible_obb", "orc_invert_transpose") else \
+                (SZ if name == "orc_result_diff" else None)
         _lib = lib
     return _lib
 
@@ -65,6 +67,7 @@ def load_ref():
         ref = C.CDLL(str(REF_LIB))
         u32p = C.POINTER(C.c_uint32)
         sig = {
+            "ref_invert_transpose": (C.c_int, [P, P]),
             "ref_vec4_mul_mat44": (None, [P, P, P]),
             "ref_mat44_mul": (None, [P, P, P]),
             "ref_scalar_mul_vec4": (None, [C.c_float, P, P]),
@@ -223,3 +226,15 @@ def result_diff(n, old_words, new_words) -> np.ndarray:
     out = np.empty(max(int(n), 1), np.uint32)
     c = load().orc_result_diff(int(n), _p(o), _p(w), _p(out))
     return out[:c].copy()
+
+
+def invert_transpose(ms):
+    """dp::math::invertTranspose over (n, 4, 4) matrices: (ok (n,) uint8, out (n, 4, 4)).
+    Where ok is 0 (det == 0) the output keeps the value of `out`'s initial fill (NaN)."""
+    m = _f32(ms).reshape(-1, 16)
+    out = np.full(m.shape, np.nan, np.float32)
+    ok = np.zeros(m.shape[0], np.uint8)
+    fn = load().orc_invert_transpose
+    for i in range(m.shape[0]):
+        ok[i] = fn(_p(m[i]), _p(out[i]))
+    return ok, out.reshape(-1, 4, 4)

@@ -88,6 +88,16 @@ void ref_mat44_mul(const float* a, const float* b, float* out)
   store_mat(load_mat(a) * load_mat(b), out);
 }
 
+// dp::math::invertTranspose (Matmnt.h:1171-1201): returns its result; out is
+// written only when it returns true (the reference leaves mOut untouched otherwise).
+int ref_invert_transpose(const float* m, float* out)
+{
+  Mat44f r = load_mat(out);
+  const bool ok = dp::math::invertTranspose(load_mat(m), r);
+  if (ok) store_mat(r, out);
+  return ok ? 1 : 0;
+}
+
 void ref_scalar_mul_vec4(float s, const float* v, float* out)
 {
   Vec4f a(v[0], v[1], v[2], v[3]);

@@ -54,7 +54,7 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_uint64), ("batches", C.c_uint64), ("tests", C.c_uint64),
                 ("delta_matrices", C.c_uint64), ("scene_bytes", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
-                ("busy_us", C.c_double)]
+                ("busy_us", C.c_double), ("issue_us", C.c_double)]
 
 
 class CullConfig(C.Structure):
@@ -121,6 +121,7 @@ _SIGNATURES = {
     "dpc_result_diff": (I, [P, P, SZ, P, P, I, P, P]),
     "dpc_bbox_workspace": (SZ, [SZ]),
     "dpc_bbox": (I, [P, P, P, SZ, P, SZ, P, P, P]),
+    "dpc_inverse_transpose": (I, [P, SZ, P, SZ, P, SZ, P, P]),
     "dpc_server_create": (I, [C.POINTER(ServerConfig), C.POINTER(P)]),
     "dpc_server_destroy": (I, [P]),
     "dpc_server_add_scene": (I, [P, P, U64, P, U64, P, C.POINTER(U32)]),

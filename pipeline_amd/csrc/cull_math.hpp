@@ -199,6 +199,46 @@ __device__ __forceinline__ RawObj load_raw(const float4* __restrict__ box_a, con
   return r;
 }
 
+// Streaming loads (read once, not kept in the caches) for data that one launch
+// reads exactly once.
+__device__ __forceinline__ float4 load_stream(const float4* p)
+{
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ float2 load_stream(const float2* p)
+{
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+  return make_float2(v.x, v.y);
+}
+
+// load_raw with streaming loads for the boxes and, when each object has a
+// matrix of its own (StreamMatrices), for the matrix rows too; a matrix shared
+// by several objects (an indexed scene) is read through the caches.
+template <bool StreamMatrices>
+__device__ __forceinline__ RawObj load_raw_stream(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
+                                                  const float4* __restrict__ planes, size_t ld, size_t i, size_t m)
+{
+  RawObj r;
+  r.a = load_stream(box_a + i);
+  r.b = load_stream(box_b + i);
+  if (StreamMatrices) {
+    r.r0 = load_stream(planes + m);
+    r.r1 = load_stream(planes + ld + m);
+    r.r2 = load_stream(planes + 2 * ld + m);
+    r.r3 = load_stream(planes + 3 * ld + m);
+  } else {
+    r.r0 = planes[m];
+    r.r1 = planes[ld + m];
+    r.r2 = planes[2 * ld + m];
+    r.r3 = planes[3 * ld + m];
+  }
+  return r;
+}
+
 // Per-lane object slot of a tile: the (clamped) indices to load from and what
 // the lane's visibility bit becomes.  An object whose matrix index lies outside
 // the matrix array reads nothing of it and is reported visible (the value the

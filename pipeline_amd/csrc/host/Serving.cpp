@@ -160,6 +160,7 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   snapshot.reserve(pick.size());
   for (auto& r : pick) snapshot.push_back(scenes[r->scene]);
   lock.unlock();
+  const double t_issue0 = now();
 
   // Arena layout: the batch descriptors (entries, pairs, frusta; staged in the
   // pinned host arena at the same offsets), then per request the delta indices
@@ -250,7 +251,9 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
     e = hipEventRecord(slot.done, s);
     if (e != hipSuccess) rc = hip_fail(e, "completion event");
   }
+  const double t_issue1 = now();
   lock.lock();
+  stats.issue_us += t_issue1 - t_issue0;
   slot.reqs = pick;
   slot.busy = true;
   slot.t_launch = t_admit;

@@ -181,7 +181,7 @@ TreeLayout::Compiled TreeLayout::compile() const
 struct Tree::Device {
   void* stream = nullptr;
   size_t ld = 0;
-  DevBuf<float> local, world, world_tmp, aos;
+  DevBuf<float> local, world, world_tmp, aos, it;
   DevBuf<uint32_t> parent_slot, index_of_slot, slot_of_index, old_slot, idx, bits;
   DevBuf<uint8_t> dirty_local, dirty_world;
   DevBuf<float> delta;
@@ -288,6 +288,7 @@ void Tree::recompile()
   d.ld = cap;
   m_compiled = std::move(next);
   m_structureDirty = false;
+  m_itAll = true;  // slots moved: recompute every inverse transpose from the relaid-out worlds
 }
 
 void Tree::compute(Mat44f const&)
@@ -321,6 +322,7 @@ void Tree::compute(Mat44f const&)
   check(dpc_tree_propagate(d.local.p, d.world.p, d.ld, d.parent_slot.p, off.data(), off.size() - 1, d.dirty_local.p,
                            d.dirty_world.p, d.stream),
         "dpc_tree_propagate");
+  if (m_itEnabled) updateInverseTranspose();
   // EventWorldMatricesChanged: the dirty-world bytes as a bitset over indices.
   const size_t words32 = 2 * ((cap + 63) / 64);
   check(dpc_fill_u32(d.bits.p, 0u, words32, d.stream), "clear bits");
@@ -334,7 +336,56 @@ void Tree::compute(Mat44f const&)
   std::fill(m_dirtyLocal.begin(), m_dirtyLocal.end(), 0);
   m_anyDirty = false;
   m_worldHostStale = true;
+  m_itHostStale = true;
 }
+
+// Runs on the tree's stream after propagation, while the dirty-world bytes
+// still mark the matrices this compute changed.
+void Tree::updateInverseTranspose()
+{
+  Device& d = *m_dev;
+  if (m_itAll) {
+    d.it.reserve(16 * d.ld);
+    check(dpc_fill_u32(reinterpret_cast<uint32_t*>(d.it.p), 0u, 16 * d.ld, d.stream), "clear inverse transposes");
+  }
+  check(dpc_inverse_transpose(d.world.p, d.ld, m_itAll ? nullptr : d.dirty_world.p, d.ld, d.it.p, d.ld, nullptr,
+                              d.stream),
+        "dpc_inverse_transpose");
+  m_itAll = false;
+}
+
+void Tree::setInverseTransposeEnabled(bool enabled)
+{
+  if (enabled && !m_itEnabled) {
+    m_itAll = true;
+    if (m_dev->ld != 0 && !m_structureDirty) {
+      // worlds exist already: compute every inverse transpose now
+      updateInverseTranspose();
+      check(dpc_stream_sync(m_dev->stream), "inverse transpose sync");
+    }
+  }
+  m_itEnabled = enabled;
+  m_itHostStale = true;
+}
+
+Mat44f const* Tree::getWorldMatricesInverseTranspose() const
+{
+  if (!m_itEnabled) throw std::logic_error("Tree::getWorldMatricesInverseTranspose: not enabled");
+  if (m_itHostStale) {
+    Device& d = *m_dev;
+    const size_t cap = m_layout.capacity();
+    m_itHost.assign(cap, Mat44f{});
+    if (d.ld == cap && d.it.p) {
+      check(dpc_matrices_to_aos(d.it.p, d.ld, d.slot_of_index.p, cap, d.aos.p, 64, d.stream), "inverse to aos");
+      check(dpc_memcpy_d2h(m_itHost.data(), d.aos.p, cap * 64, d.stream), "download inverse transposes");
+      check(dpc_stream_sync(d.stream), "download sync");
+    }
+    m_itHostStale = false;
+  }
+  return m_itHost.data();
+}
+
+const float* Tree::deviceWorldInverseTransposePlanes() const { return m_itEnabled ? m_dev->it.p : nullptr; }
 
 Mat44f const* Tree::getWorldMatrices() const
 {

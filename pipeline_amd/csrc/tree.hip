@@ -25,7 +25,7 @@ namespace {
 
 constexpr unsigned kTreeThreads = 256;
 constexpr unsigned kSmallThreads = 1024;
-constexpr unsigned kLdsNodes = 2048;  // levels up to this size may share a single-workgroup launch
+constexpr unsigned kLdsNodes = 1024;  // slots of one single-workgroup run (all of them staged in LDS)
 constexpr int kPerThread = kLdsNodes / kSmallThreads;
 constexpr int kMaxLevelsPerSmall = 32;
 static_assert(kLdsNodes % kSmallThreads == 0, "slots per thread");
@@ -68,17 +68,21 @@ struct SlotIn {
   uint32_t dl;
 };
 
+template <bool HasDirty>
 __device__ __forceinline__ void load_level(const float4* __restrict__ L, size_t ld, const uint32_t* __restrict__ parent,
                                            const uint8_t* __restrict__ dirty_local, uint64_t a, uint64_t b,
                                            SlotIn (&in)[kPerThread])
 {
-  if (a >= b) return;  // an empty level: nothing to read (a may equal ld)
+  // Unconditional and clamped: lanes past the level (and an empty level) read a
+  // slot that exists (slot 0, the root, always does) and ignore it.  Loads
+  // that every path issues let the compiler wait for exactly the loads a level
+  // needs.
 #pragma unroll
   for (int k = 0; k < kPerThread; ++k) {
     const uint64_t s = a + threadIdx.x + static_cast<uint64_t>(k) * kSmallThreads;
-    const uint64_t c = s < b ? s : a;  // clamped: a slot of the level, read and then ignored
+    const uint64_t c = s < b ? s : (b > a ? a : 0);
     in[k].p = parent[c];
-    in[k].dl = dirty_local ? dirty_local[c] : 1u;
+    in[k].dl = HasDirty ? dirty_local[c] : 1u;
     in[k].l0 = L[c];
     in[k].l1 = L[ld + c];
     in[k].l2 = L[2 * ld + c];
@@ -86,12 +90,90 @@ __device__ __forceinline__ void load_level(const float4* __restrict__ L, size_t 
   }
 }
 
-// One workgroup walks a run of small levels (each at most kLdsNodes slots; the
-// host guarantees it).  LDS holds the previous level's world rows and dirty
-// flags; a parent outside that level (the first level of the run, or a level
-// split off after index reuse) is read from memory, where this workgroup or an
-// earlier launch wrote it.  The barriers order one level's stores before the
-// next level's loads (all waves of a workgroup share one CU).
+// Orders one level's LDS stores before the next level's LDS loads.  Global
+// memory needs no ordering inside a run: every parent the run reads from memory
+// lies outside the slots the run writes before that read (it was written by an
+// earlier launch, or is an unlisted slot or a slot of a later pass of the same
+// level, whose old value the reference reads too), so the wait covers LDS only
+// and the next level's prefetched loads stay in flight across it.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// A parent that is not staged in LDS, read from memory.  The empty asm takes
+// the loaded values as register operands: it keeps the compiler from folding
+// this load and the LDS read of the other branch into one generic-address load,
+// and it places the wait for these loads inside this (rarely taken) branch
+// rather than where the branches join.
+__device__ __forceinline__ void load_parent(const float4* W, size_t ld, const uint8_t* dirty_world, uint64_t p,
+                                            float4& p0, float4& p1, float4& p2, float4& p3, uint32_t& dp)
+{
+  p0 = W[p];
+  p1 = W[ld + p];
+  p2 = W[2 * ld + p];
+  p3 = W[3 * ld + p];
+  dp = dirty_world[p];
+  asm volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(p0.z), "+v"(p0.w), "+v"(p1.x), "+v"(p1.y), "+v"(p1.z), "+v"(p1.w));
+  asm volatile("" : "+v"(p2.x), "+v"(p2.y), "+v"(p2.z), "+v"(p2.w), "+v"(p3.x), "+v"(p3.y), "+v"(p3.z), "+v"(p3.w),
+               "+v"(dp));
+}
+
+// Computes one level of a run from its prefetched inputs into LDS: the rows of
+// every matrix the level recomputes and one flag per slot (1 = recomputed).  A
+// parent outside the run, or one the run left unchanged, is read from memory,
+// where it keeps its value for the whole launch.  Nothing is stored to memory
+// here (tree_small_levels_kernel writes the run out at its end), so the only
+// vector-memory operations in flight are loads, and waiting for the current
+// level's inputs does not wait for the next level's prefetch.
+__device__ __forceinline__ void run_level(const SlotIn (&in)[kPerThread], const float4* W, size_t ld,
+                                          const uint8_t* dirty_world, uint64_t base, uint64_t a, uint64_t b,
+                                          float4 (&lds_w)[4][kLdsNodes], uint8_t (&lds_d)[kLdsNodes])
+{
+#pragma unroll
+  for (int k = 0; k < kPerThread; ++k) {
+    const uint64_t s = a + threadIdx.x + static_cast<uint64_t>(k) * kSmallThreads;
+    if (s >= b) continue;
+    const uint64_t p = in[k].p;
+    const bool valid = p < ld;  // a parent outside the arrays is never read (as in propagate_slot)
+    const bool staged = valid && p >= base && p < a;
+    const uint64_t i = staged ? p - base : 0;
+    const uint32_t dp_lds = staged ? lds_d[i] : 0u;
+    bool dirty = false;
+    float4 p0, p1, p2, p3;
+    if (dp_lds) {  // the parent was recomputed in this run: its rows are in LDS
+      p0 = lds_w[0][i];
+      p1 = lds_w[1][i];
+      p2 = lds_w[2][i];
+      p3 = lds_w[3][i];
+      dirty = true;
+    } else if (valid && (!staged || in[k].dl)) {
+      uint32_t dp = 0;
+      load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
+      // a staged parent that the run left unchanged has dirty_world == 0
+      dirty = (in[k].dl | (staged ? 0u : dp)) != 0;
+    }
+    const uint64_t j = s - base;
+    if (dirty) {
+      lds_w[0][j] = rowmat(in[k].l0, p0, p1, p2, p3);
+      lds_w[1][j] = rowmat(in[k].l1, p0, p1, p2, p3);
+      lds_w[2][j] = rowmat(in[k].l2, p0, p1, p2, p3);
+      lds_w[3][j] = rowmat(in[k].l3, p0, p1, p2, p3);
+    }
+    lds_d[j] = dirty ? 1 : 0;
+  }
+}
+
+// One workgroup walks a run of small levels whose slots total at most
+// kLdsNodes (the host guarantees it).  Every matrix the run recomputes is kept
+// in LDS, at its slot's offset in the run, where the later levels of the run
+// read their parents; a parent outside the run comes from memory.  Levels go in
+// pairs over two register sets, so that the loads of the next level are issued
+// before the current one is computed and no register copy waits for them.  The
+// recomputed rows and dirty flags go to memory once, after the last level.
+template <bool HasDirty>
 __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const float4* __restrict__ L, float4* W,
                                                                           size_t ld,
                                                                           const uint32_t* __restrict__ parent,
@@ -101,80 +183,92 @@ __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const 
 {
   __shared__ float4 lds_w[4][kLdsNodes];
   __shared__ uint8_t lds_d[kLdsNodes];
-  uint64_t prev0 = 0, prev1 = 0;  // the slots whose rows LDS holds
-  SlotIn cur[kPerThread];
-  load_level(L, ld, parent, dirty_local, run.off[0], run.off[1], cur);
-  for (int l = 0; l < run.n; ++l) {
-    const uint64_t a = run.off[l], b = run.off[l + 1];
-    SlotIn nxt[kPerThread];
-    if (l + 1 < run.n) load_level(L, ld, parent, dirty_local, run.off[l + 1], run.off[l + 2], nxt);
-    float4 w[kPerThread][4];
-    uint32_t dw[kPerThread];
-#pragma unroll
-    for (int k = 0; k < kPerThread; ++k) {
-      const uint64_t s = a + threadIdx.x + static_cast<uint64_t>(k) * kSmallThreads;
-      dw[k] = 0;
-      if (s >= b) continue;
-      const uint64_t p = cur[k].p;
-      if (p >= ld) {  // never read (as in propagate_slot); LDS gets the slot's current rows
-        w[k][0] = W[s];
-        w[k][1] = W[ld + s];
-        w[k][2] = W[2 * ld + s];
-        w[k][3] = W[3 * ld + s];
-        dw[k] = dirty_world[s];
-        continue;
-      }
-      const bool staged = p >= prev0 && p < prev1;
-      const uint32_t dp = staged ? lds_d[p - prev0] : dirty_world[p];
-      if (cur[k].dl | dp) {
-        float4 p0, p1, p2, p3;
-        if (staged) {
-          p0 = lds_w[0][p - prev0];
-          p1 = lds_w[1][p - prev0];
-          p2 = lds_w[2][p - prev0];
-          p3 = lds_w[3][p - prev0];
-        } else {
-          p0 = W[p];
-          p1 = W[ld + p];
-          p2 = W[2 * ld + p];
-          p3 = W[3 * ld + p];
-        }
-        w[k][0] = rowmat(cur[k].l0, p0, p1, p2, p3);
-        w[k][1] = rowmat(cur[k].l1, p0, p1, p2, p3);
-        w[k][2] = rowmat(cur[k].l2, p0, p1, p2, p3);
-        w[k][3] = rowmat(cur[k].l3, p0, p1, p2, p3);
-        W[s] = w[k][0];
-        W[ld + s] = w[k][1];
-        W[2 * ld + s] = w[k][2];
-        W[3 * ld + s] = w[k][3];
-        dirty_world[s] = 1;
-        dw[k] = 1;
-      } else {  // clean: the world matrix is unchanged; its children may still need it
-        w[k][0] = W[s];
-        w[k][1] = W[ld + s];
-        w[k][2] = W[2 * ld + s];
-        w[k][3] = W[3 * ld + s];
-        dw[k] = dirty_world[s];
-      }
-    }
-    __syncthreads();  // every wave is done reading the previous level from LDS (and its stores are visible)
-#pragma unroll
-    for (int k = 0; k < kPerThread; ++k) {
-      const uint64_t s = a + threadIdx.x + static_cast<uint64_t>(k) * kSmallThreads;
-      if (s >= b) continue;
-      const uint64_t i = s - a;
-      lds_w[0][i] = w[k][0];
-      lds_w[1][i] = w[k][1];
-      lds_w[2][i] = w[k][2];
-      lds_w[3][i] = w[k][3];
-      lds_d[i] = static_cast<uint8_t>(dw[k]);
-    }
-    prev0 = a;
-    prev1 = b;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < kPerThread; ++k) cur[k] = nxt[k];
+  const uint64_t base = run.off[0];
+  const int last = run.n - 1;
+  SlotIn A[kPerThread], B[kPerThread];
+  load_level<HasDirty>(L, ld, parent, dirty_local, run.off[0], run.off[1], A);
+  for (int l = 0; l < run.n; l += 2) {
+    // the next level's inputs (past the last level: the last level again, unused)
+    const int lb = l + 1 < run.n ? l + 1 : last;
+    load_level<HasDirty>(L, ld, parent, dirty_local, run.off[lb], run.off[lb + 1], B);
+    run_level(A, W, ld, dirty_world, base, run.off[l], run.off[l + 1], lds_w, lds_d);
+    lds_barrier();
+    if (l + 1 >= run.n) break;
+    const int la = l + 2 < run.n ? l + 2 : last;
+    load_level<HasDirty>(L, ld, parent, dirty_local, run.off[la], run.off[la + 1], A);
+    run_level(B, W, ld, dirty_world, base, run.off[l + 1], run.off[l + 2], lds_w, lds_d);
+    lds_barrier();
   }
+  const uint64_t total = run.off[run.n] - base;
+  for (uint64_t j = threadIdx.x; j < total; j += kSmallThreads) {
+    if (!lds_d[j]) continue;
+    const uint64_t s = base + j;
+    W[s] = lds_w[0][j];
+    W[ld + s] = lds_w[1][j];
+    W[2 * ld + s] = lds_w[2][j];
+    W[3 * ld + s] = lds_w[3][j];
+    dirty_world[s] = 1;
+  }
+}
+
+// dp::math::invertTranspose (dp/math/Matmnt.h:1171-1201) of W[s]: 2x2 minors of
+// rows 0-1 (s*) and rows 2-3 (c*), the determinant, one correctly rounded
+// division, then each cofactor sum times 1/det, evaluated left to right as the
+// reference writes it.  Output row i, column j = the reference's mOut[i][j].
+__device__ __forceinline__ bool invert_transpose(const float4 r0, const float4 r1, const float4 r2, const float4 r3,
+                                                 float4& o0, float4& o1, float4& o2, float4& o3)
+{
+  const float m00 = r0.x, m01 = r0.y, m02 = r0.z, m03 = r0.w;
+  const float m10 = r1.x, m11 = r1.y, m12 = r1.z, m13 = r1.w;
+  const float m20 = r2.x, m21 = r2.y, m22 = r2.z, m23 = r2.w;
+  const float m30 = r3.x, m31 = r3.y, m32 = r3.z, m33 = r3.w;
+  const float s0 = m00 * m11 - m01 * m10, c5 = m22 * m33 - m23 * m32;
+  const float s1 = m00 * m12 - m02 * m10, c4 = m21 * m33 - m23 * m31;
+  const float s2 = m00 * m13 - m03 * m10, c3 = m21 * m32 - m22 * m31;
+  const float s3 = m01 * m12 - m02 * m11, c2 = m20 * m33 - m23 * m30;
+  const float s4 = m01 * m13 - m03 * m11, c1 = m20 * m32 - m22 * m30;
+  const float s5 = m02 * m13 - m03 * m12, c0 = m20 * m31 - m21 * m30;
+  const float det = ((((s0 * c5 - s1 * c4) + s2 * c3) + s3 * c2) - s4 * c1) + s5 * c0;
+  if (!(det != 0.0f)) return false;
+  const float inv = 1.0f / det;
+  o0.x = ((m11 * c5 - m12 * c4) + m13 * c3) * inv;
+  o1.x = ((-m01 * c5 + m02 * c4) - m03 * c3) * inv;
+  o2.x = ((m31 * s5 - m32 * s4) + m33 * s3) * inv;
+  o3.x = ((-m21 * s5 + m22 * s4) - m23 * s3) * inv;
+  o0.y = ((-m10 * c5 + m12 * c2) - m13 * c1) * inv;
+  o1.y = ((m00 * c5 - m02 * c2) + m03 * c1) * inv;
+  o2.y = ((-m30 * s5 + m32 * s2) - m33 * s1) * inv;
+  o3.y = ((m20 * s5 - m22 * s2) + m23 * s1) * inv;
+  o0.z = ((m10 * c4 - m11 * c2) + m13 * c0) * inv;
+  o1.z = ((-m00 * c4 + m01 * c2) - m03 * c0) * inv;
+  o2.z = ((m30 * s4 - m31 * s2) + m33 * s0) * inv;
+  o3.z = ((-m20 * s4 + m21 * s2) - m23 * s0) * inv;
+  o0.w = ((-m10 * c3 + m11 * c1) - m12 * c0) * inv;
+  o1.w = ((m00 * c3 - m01 * c1) + m02 * c0) * inv;
+  o2.w = ((-m30 * s3 + m31 * s1) - m32 * s0) * inv;
+  o3.w = ((m20 * s3 - m21 * s1) + m22 * s0) * inv;
+  return true;
+}
+
+// One thread per slot: reads W[s] (4 coalesced row loads), writes the inverse
+// transpose in the same row-plane layout.  64 B in + 64 B out per matrix.
+__global__ __launch_bounds__(kTreeThreads) void inverse_transpose_kernel(const float4* __restrict__ W, size_t ld,
+                                                                         const uint8_t* __restrict__ dirty, size_t n,
+                                                                         float4* __restrict__ out, size_t out_ld,
+                                                                         uint8_t* __restrict__ ok)
+{
+  const size_t s = static_cast<size_t>(blockIdx.x) * kTreeThreads + threadIdx.x;
+  if (s >= n) return;
+  if (dirty && dirty[s] == 0) return;
+  float4 o0, o1, o2, o3;
+  const bool good = invert_transpose(W[s], W[ld + s], W[2 * ld + s], W[3 * ld + s], o0, o1, o2, o3);
+  if (good) {
+    out[s] = o0;
+    out[out_ld + s] = o1;
+    out[2 * out_ld + s] = o2;
+    out[3 * out_ld + s] = o3;
+  }
+  if (ok) ok[s] = good ? 1 : 0;
 }
 
 __global__ void bytes_to_bits_kernel(const uint8_t* __restrict__ bytes, const uint32_t* __restrict__ index_of_slot,
@@ -216,16 +310,22 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
   while (l < n_levels) {
     const uint64_t size = h_level_offsets[l + 1] - h_level_offsets[l];
     if (size <= kLdsNodes) {
+      // a run: consecutive small levels while their slots total at most kLdsNodes
       LevelRun run;
       run.n = 0;
       run.off[0] = h_level_offsets[l];
-      while (l < n_levels && run.n < kMaxLevelsPerSmall && h_level_offsets[l + 1] - h_level_offsets[l] <= kLdsNodes) {
+      while (l < n_levels && run.n < kMaxLevelsPerSmall &&
+             h_level_offsets[l + 1] - run.off[0] <= kLdsNodes) {
         run.off[run.n + 1] = h_level_offsets[l + 1];
         ++run.n;
         ++l;
       }
-      hipLaunchKernelGGL(tree_small_levels_kernel, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld, d_parent_slot, run,
-                         d_dirty_local, d_dirty_world);
+      if (d_dirty_local)
+        hipLaunchKernelGGL(tree_small_levels_kernel<true>, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld, d_parent_slot,
+                           run, d_dirty_local, d_dirty_world);
+      else
+        hipLaunchKernelGGL(tree_small_levels_kernel<false>, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld,
+                           d_parent_slot, run, d_dirty_local, d_dirty_world);
       int rc = check_launch("tree_small_levels_kernel");
       if (rc) return rc;
     } else {
@@ -238,6 +338,21 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
     }
   }
   return DPC_OK;
+}
+
+int dpc_inverse_transpose(const float* d_planes, size_t ld, const uint8_t* d_dirty, size_t n, float* d_out_planes,
+                          size_t out_ld, uint8_t* d_ok, void* stream)
+{
+  if (n == 0) return DPC_OK;
+  if (!d_planes || !d_out_planes) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_inverse_transpose: null pointer");
+  if (n > ld || n > out_ld)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_inverse_transpose: n=%zu exceeds ld=%zu or out_ld=%zu", n, ld, out_ld);
+  if (!aligned(d_planes, 16) || !aligned(d_out_planes, 16))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_inverse_transpose: misaligned matrix planes");
+  hipLaunchKernelGGL(inverse_transpose_kernel, dim3(static_cast<unsigned>((n + kTreeThreads - 1) / kTreeThreads)),
+                     dim3(kTreeThreads), 0, as_stream(stream), reinterpret_cast<const float4*>(d_planes), ld, d_dirty, n,
+                     reinterpret_cast<float4*>(d_out_planes), out_ld, d_ok);
+  return check_launch("inverse_transpose_kernel");
 }
 
 int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of_slot, size_t n_slots,

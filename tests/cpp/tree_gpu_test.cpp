@@ -137,7 +137,12 @@ static void random_frames()
   }
   Recorder rec;
   tree.attach(&rec);
+  tree.setInverseTransposeEnabled(true);
   for (int frame = 0; frame < 4; ++frame) {
+    if (frame == 2) {  // off and on again with worlds present: every inverse transpose is recomputed at once
+      tree.setInverseTransposeEnabled(false);
+      tree.setInverseTransposeEnabled(true);
+    }
     if (frame) {
       for (int k = 0; k < 500; ++k) {
         const Index i = nodes[rng() % nodes.size()];
@@ -158,7 +163,16 @@ static void random_frames()
       changed += ev[i];
     }
     EXPECT(bad == 0, "frame %d: %zu world matrices or event bits differ", frame, bad);
-    std::printf("  frame %d: %zu nodes, %zu recomputed, %zu mismatches\n", frame, nodes.size(), changed, bad);
+    // inverse transposes of the world matrices (dp::math::invertTranspose), bit for bit
+    const Mat44f* it = tree.getWorldMatricesInverseTranspose();
+    size_t bad_it = 0;
+    for (Index i : nodes) {
+      float want[16];
+      if (orc_invert_transpose(&world[16 * i], want) && std::memcmp(it[i].m, want, 64) != 0) ++bad_it;
+    }
+    EXPECT(bad_it == 0, "frame %d: %zu inverse transposes differ", frame, bad_it);
+    std::printf("  frame %d: %zu nodes, %zu recomputed, %zu mismatches, %zu inverse-transpose mismatches\n", frame,
+                nodes.size(), changed, bad, bad_it);
   }
 }
 

@@ -4,7 +4,7 @@ dp::util::BitArray, dp::transform::Tree, compiled from /root/reference by
 oracle/Makefile).
 
 Run in the container that has the reference sources:
-    make -C oracle ref && python tests/golden/gen_ref_fixtures.py
+    make -C oracle ref && python tests/golden/gen_ref_fixtures.py [math bitarray tree invert]
 
 The fixtures are data only (inputs and the reference's outputs); the reference
 itself is not needed to check against them.
@@ -232,12 +232,63 @@ def write_tree_trace_bin(arrays):
     (OUT / "ref_tree_trace.bin").write_bytes(bytes(out))
 
 
+def gen_invert(ref, rng):
+    """dp::math::invertTranspose on world-like matrices (products of rotation x
+    scale x translation chains), on matrices with entries over a wide exponent
+    range, and on singular, non-finite and underflowing cases."""
+    from pipeline_amd import scenes
+    cases = []
+    # chains of 1-6 local TRS matrices, multiplied by the reference's Mat44 product
+    for depth in range(1, 7):
+        loc = scenes.trs_matrices(rng, 300 * depth, scale=(0.8, 1.25), trans=(-10, 10)).reshape(-1, depth, 16)
+        for chain in loc:
+            w = chain[0].copy()
+            for k in range(1, depth):
+                nxt = np.empty(16, np.float32)
+                ref.ref_mat44_mul(_p(chain[k]), _p(w), _p(nxt))
+                w = nxt
+            cases.append(w)
+    cases += list(wide_floats(rng, (1500, 16)))
+    special = []
+    base = wide_floats(rng, (40, 16)).reshape(-1, 4, 4)
+    for b in base[:10]:  # two equal rows: every 2x2 minor of rows 0,1 is exactly 0
+        b = b.copy()
+        b[1] = b[0]
+        special.append(b.reshape(16))
+    for b in base[10:20]:  # a zero column
+        b = b.copy()
+        b[:, 2] = 0
+        special.append(b.reshape(16))
+    special.append(np.zeros(16, np.float32))
+    for b in base[20:30]:  # a NaN or an infinity somewhere
+        b = b.copy().reshape(16)
+        b[int(rng.integers(16))] = np.nan if rng.random() < 0.5 else np.inf
+        special.append(b)
+    for sc in (1e-10, 1e-12, 1e-20, 1e-30, 1e15, 1e20):  # det underflows or overflows
+        special.append((np.eye(4, dtype=np.float64) * sc).astype(np.float32).reshape(16))
+        m = scenes.trs_matrices(rng, 1)[0].astype(np.float64)
+        m[:3, :3] *= sc
+        special.append(m.astype(np.float32).reshape(16))
+    cases += special
+    m = np.ascontiguousarray(np.stack(cases), np.float32)
+    out = np.full(m.shape, np.nan, np.float32)
+    ok = np.zeros(m.shape[0], np.uint8)
+    for i in range(m.shape[0]):
+        ok[i] = ref.ref_invert_transpose(_p(m[i]), _p(out[i]))
+    np.savez_compressed(OUT / "ref_invert.npz", m=m, ok=ok, out=out)
+
+
+GENERATORS = {"math": gen_math, "bitarray": gen_bitarray, "tree": gen_tree, "invert": gen_invert}
+
+
 def main():
     ref = orc.load_ref()
+    which = sys.argv[1:] or list(GENERATORS)
     rng = np.random.default_rng(20261202)
-    gen_math(ref, rng)
-    gen_bitarray(ref, rng)
-    gen_tree(ref, rng)
+    for name, gen in GENERATORS.items():
+        # every set draws from its own stream, so regenerating one leaves the others as they are
+        if name in which:
+            gen(ref, np.random.default_rng([20261202, list(GENERATORS).index(name)]) if name == "invert" else rng)
     for f in sorted(OUT.glob("ref_*.npz")):
         print(f.name, f.stat().st_size, "bytes")
 

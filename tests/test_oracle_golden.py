@@ -159,3 +159,36 @@ def test_tree_trace_covers_orphans_and_reuse():
     assert removed & re_added                                # freed indices were reused
     assert ((ops[:, 0] == 0) & (ops[:, 2] == 0xFFFFFFFF)).any()   # addTransform with a bad parent threw
     assert ((ops[:, 0] == 2) & (ops[:, 2] == 0xFFFFFFFF)).any()   # removeTransform of a bad index threw
+
+
+def same_bits_or_both_nan(a, b) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return (a.view(np.uint32) == b.view(np.uint32)) | (np.isnan(a) & np.isnan(b))
+
+
+def test_invert_transpose_matches_reference(orc):
+    """orc_invert_transpose against dp::math::invertTranspose (Matmnt.h:1171-1201)
+    compiled from the reference: same success flag, same bits (any NaN equals any NaN)."""
+    fx = np.load(GOLDEN / "ref_invert.npz")
+    ok, out = orc.invert_transpose(fx["m"])
+    assert np.array_equal(ok, fx["ok"])
+    good = ok == 1
+    assert same_bits_or_both_nan(out[good].reshape(-1, 16), fx["out"][good]).all()
+    assert 0 < (ok == 0).sum() < 100  # the singular cases are in the set
+
+
+def test_invert_transpose_is_the_inverse_transposed(orc):
+    """What the mathematics fixes: M @ result^T = I for well-conditioned matrices
+    (a transposed or sign-flipped cofactor fails this), and det == 0 returns 0."""
+    from pipeline_amd import scenes
+    rng = np.random.default_rng(7)
+    m = scenes.trs_matrices(rng, 200, scale=(0.5, 2.0), trans=(-10, 10))
+    ok, out = orc.invert_transpose(m)
+    assert ok.all()
+    prod = m.astype(np.float64) @ np.transpose(out.astype(np.float64), (0, 2, 1))
+    assert np.abs(prod - np.eye(4)).max() < 1e-5
+    sing = m[:5].copy()
+    sing[:, 3] = sing[:, 2]  # two equal rows
+    ok, out = orc.invert_transpose(sing)
+    assert not ok.any() and np.isnan(out).all()  # untouched
