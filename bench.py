@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="launch only a 512 MiB device copy, the batched kernel and the flat kernel a few times "
                          "each (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes); prints no JSON")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
+    ap.add_argument("--serve-streams", type=int, default=4, help="batches in flight in the serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of each serving-rate run")
     ap.add_argument("--tune", default="", help="batch config k,block,waves_per_cu,max_frusta")
     return ap.parse_args()
@@ -146,7 +148,8 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     log(f"serving: pinned copies {pcie['h2d_gbs']:.1f} GB/s host->device, {pcie['d2h_gbs']:.1f} GB/s device->host")
     n_slots, slot_words = 192, 32 * ((max(sizes) + 63) // 64)
     out_pool = PinnedArray((n_slots, slot_words), np.uint64)
-    srv = Server(max_scenes=16, max_batch_requests=16, streams=4, hbm_budget_bytes=32 << 30)
+    streams = getattr(args, "serve_streams", 4)
+    srv = Server(max_scenes=16, max_batch_requests=16, streams=streams, hbm_budget_bytes=32 << 30)
     ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scs]
     base_mats = [sc.matrices.reshape(-1, 16).copy() for sc in scs]
     runs = []
@@ -254,7 +257,7 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     for p in pinned:
         p.free()
     return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check, pinned_copy=pcie,
-                server={"streams": 4, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
+                server={"streams": streams, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
                         "scene_bytes": st["scene_bytes"], "peak_inflight_bytes": st["peak_inflight_bytes"]})
 
 
@@ -319,6 +322,11 @@ def main():
     args = parse()
     if args.pmc_probe:
         pmc_probe()
+        return
+    if args.serve_only:
+        from oracle import oracle as orc
+        from pipeline_amd import _lib
+        print(json.dumps(bench_serving(args, _lib.load(), orc)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}); "

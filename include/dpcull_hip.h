@@ -245,17 +245,19 @@ int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
  * batching: requests that arrive while batches run join the next free stream).
  * A request = apply a matrix delta to its scene, then cull the whole scene
  * against 1..64 frusta.  Requests of one scene run in submission order, one per
- * batch; requests are admitted while the device memory they need fits the
- * budget left after the resident scenes. */
+ * batch, on one stream while any of them is in flight; requests are admitted
+ * while the device memory they need fits the budget left after the resident
+ * scenes (a quarter of the budget, at most 1 GiB, is the in-flight arena,
+ * split over streams x batches_per_stream batches). */
 typedef struct dpc_server dpc_server;
 
 typedef struct dpc_server_config {
   uint32_t max_scenes;          /* resident scenes (<= 64) */
   uint32_t max_batch_requests;  /* requests per launch (<= 64) */
-  uint32_t streams;             /* batches in flight at once (1..8) */
+  uint32_t streams;             /* ordering domains (HIP streams), 1..8 */
   uint32_t objects_per_lane;    /* kernel shape: 1 or 2 */
   uint32_t waves_per_cu;        /* kernel shape: persistent waves per CU */
-  uint32_t reserved;
+  uint32_t batches_per_stream;  /* batches in flight per stream (0: 2, at most 4) */
   uint64_t hbm_budget_bytes;    /* resident scenes + in-flight requests */
 } dpc_server_config;
 

@@ -16,25 +16,28 @@
 namespace dpc {
 namespace {
 
-constexpr unsigned kCullThreads = 256;  // persistent streaming form
-constexpr unsigned kPlainThreads = 512;  // plain form
+constexpr unsigned kCullThreads = 256;  // persistent streaming form; plain form with an index array
+constexpr unsigned kPlainThreads = 512;  // plain form without an index array
 
 // Plain form: one object per lane, one 64-object tile per wave, a grid that
 // covers all tiles.  The object's four matrix rows come from four row planes,
 // so every load of the wave is one contiguous 1 KiB (rows, box_a) or 512 B
 // (box_b) segment; the visibility bits leave as one ballot word per wave.
-// Boxes (and, without an index array, matrices) are read once per launch, so
-// they are loaded as streaming data.
+// Without an index array every box and matrix is read once per launch, so
+// they are loaded as streaming data in 512-thread blocks; with one (matrices
+// shared by several objects, e.g. the leaves of a transform tree), plain cached
+// loads in 256-thread blocks measured faster.
 template <bool Indexed>
-__global__ __launch_bounds__(kPlainThreads) void cull_kernel(const float4* __restrict__ box_a,
+__global__ __launch_bounds__(Indexed ? kCullThreads : kPlainThreads) void cull_kernel(const float4* __restrict__ box_a,
                                                             const float2* __restrict__ box_b,
                                                             const uint32_t* __restrict__ xform, size_t n,
                                                             const float4* __restrict__ planes, size_t ld, Mat16 vp,
                                                             unsigned long long* __restrict__ mask)
 {
-  const size_t i = static_cast<size_t>(blockIdx.x) * kPlainThreads + threadIdx.x;
+  const size_t i = static_cast<size_t>(blockIdx.x) * (Indexed ? kCullThreads : kPlainThreads) + threadIdx.x;
   const Slot s = make_slot(i, n, Indexed ? xform : nullptr, ld);
-  const RawObj r = load_raw_stream<!Indexed>(box_a, box_b, planes, ld, s.i, s.m);
+  const RawObj r = Indexed ? load_raw(box_a, box_b, planes, ld, s.i, s.m)
+                           : load_raw_stream<true>(box_a, box_b, planes, ld, s.i, s.m);
   const bool vis = slot_visible(s, obb_visible(obb_of(r), vp.m));
   const unsigned long long bits = __ballot(vis);
   if ((threadIdx.x & (kWave - 1)) == 0 && i < n) mask[i >> 6] = bits;
@@ -417,13 +420,12 @@ int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
   const dpc_cull_config cfg = current_cull_config();
   const size_t tiles = (n_objects + kWave - 1) / kWave;
   if (cfg.waves_per_cu == 0) {
-    const dim3 grid(grid_for(n_objects, kPlainThreads));
     if (d_xform)
-      hipLaunchKernelGGL(cull_kernel<true>, grid, dim3(kPlainThreads), 0, as_stream(stream), a, b, d_xform, n_objects,
-                         p, ld, m, out);
+      hipLaunchKernelGGL(cull_kernel<true>, dim3(grid_for(n_objects, kCullThreads)), dim3(kCullThreads), 0,
+                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
     else
-      hipLaunchKernelGGL(cull_kernel<false>, grid, dim3(kPlainThreads), 0, as_stream(stream), a, b, d_xform,
-                         n_objects, p, ld, m, out);
+      hipLaunchKernelGGL(cull_kernel<false>, dim3(grid_for(n_objects, kPlainThreads)), dim3(kPlainThreads), 0,
+                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
   } else {
     const size_t waves_per_block = kCullThreads / kWave;
     size_t blocks = static_cast<size_t>(device_cu_count()) * cfg.waves_per_cu / waves_per_block;

@@ -3,14 +3,17 @@
 //
 // Scenes are uploaded once and stay in device memory.  Callers submit requests
 // (scene, frusta, matrix delta) from any thread; a scheduler thread forms
-// batches continuously: whenever one of its streams is free it takes, in
-// arrival order, at most one request per scene (requests of one scene keep
-// their order: each sees the deltas of the earlier ones) whose scene is not in
-// a batch still running, until the batch's device arena is full.  A batch
-// copies the deltas to the device and scatters them into the scenes' matrices,
-// then tests every scene of the batch against its request's frusta in one
-// launch (dpc::launch_serve), then copies the masks back.  Batches on different
-// streams overlap on the device.
+// batches continuously.  Each stream holds up to batches_per_stream batches in
+// flight, each with an arena of its own.  Whenever an arena is free, the
+// scheduler takes for its stream, in arrival order, at most one request per
+// scene, until the arena is full.  A scene with requests in flight is bound to
+// the stream that runs them and joins only batches of that stream: stream order
+// then keeps each scene's requests in submission order (each sees the deltas
+// of the earlier ones) without waiting for them to finish.  A batch copies the
+// deltas to the device and scatters them into the scenes' matrices, tests every
+// scene of the batch against its request's frusta in one launch
+// (dpc::launch_serve), then copies each request's masks to its caller's
+// buffer.  Batches on different streams overlap on the device.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -44,7 +47,9 @@ uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 struct Scene {
   bool live = false;
-  bool busy = false;  // in a batch that has not completed
+  bool ready = false;     // uploaded: may be scheduled
+  uint32_t inflight = 0;  // requests in batches that have not completed
+  int stream = -1;        // the stream those batches run on (-1: none)
   uint64_t n = 0, n_mat = 0, bytes = 0;
   float* box_a = nullptr;
   float* box_b = nullptr;
@@ -70,8 +75,10 @@ struct Request {
   bool done = false;
 };
 
+// One batch in flight: its arena, completion event and requests.
 struct StreamSlot {
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // borrowed from dpc_server::streams
+  int stream_index = 0;
   hipEvent_t done = nullptr;
   char* d_arena = nullptr;
   char* h_arena = nullptr;  // pinned, kHostArena bytes
@@ -95,6 +102,7 @@ struct dpc_server {
   std::unordered_map<uint64_t, std::shared_ptr<Request>> requests;
   std::vector<Scene> scenes;
   dpc_scene_desc* d_scene_table = nullptr;
+  std::vector<hipStream_t> streams;
   std::vector<StreamSlot> slots;
   uint64_t next_id = 1;
   uint32_t batch_seq = 0;
@@ -135,7 +143,8 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   for (auto it = pending.begin(); it != pending.end() && pick.size() < cfg.max_batch_requests;) {
     const auto& r = *it;
     Scene& sc = scenes[r->scene];
-    if (sc.busy || scene_taken[r->scene]) {
+    const bool other_stream = sc.inflight != 0 && sc.stream != slot.stream_index;
+    if (!sc.ready || other_stream || scene_taken[r->scene]) {
       scene_taken[r->scene] = true;  // later requests of this scene wait behind this one
       ++it;
       continue;
@@ -150,13 +159,14 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   const uint32_t batch = ++batch_seq;
   const double t_admit = now();
   for (auto& r : pick) {
-    scenes[r->scene].busy = true;
+    scenes[r->scene].inflight++;
+    scenes[r->scene].stream = slot.stream_index;
     scenes[r->scene].pending--;
     r->t_admit = t_admit;
     r->batch = batch;
     r->batch_requests = static_cast<uint32_t>(pick.size());
   }
-  std::vector<Scene> snapshot;  // scene arrays are stable while busy
+  std::vector<Scene> snapshot;  // scene arrays are stable while requests are in flight
   snapshot.reserve(pick.size());
   for (auto& r : pick) snapshot.push_back(scenes[r->scene]);
   lock.unlock();
@@ -278,7 +288,8 @@ void dpc_server::complete(StreamSlot& slot)
     if (r.status == DPC_OK) stats.delta_matrices += r.n_delta;
     r.t_done = t;
     r.done = true;
-    scenes[r.scene].busy = false;
+    Scene& sc = scenes[r.scene];
+    if (--sc.inflight == 0) sc.stream = -1;
     stats.requests_done++;
   }
   stats.batches++;
@@ -332,7 +343,7 @@ int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
   if (!cfg || !out) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_create: null argument");
   *out = nullptr;
   if (cfg->max_scenes == 0 || cfg->max_scenes > kMaxScenes || cfg->max_batch_requests == 0 ||
-      cfg->max_batch_requests > kMaxBatch || cfg->streams == 0 || cfg->streams > 8 ||
+      cfg->max_batch_requests > kMaxBatch || cfg->streams == 0 || cfg->streams > 8 || cfg->batches_per_stream > 4 ||
       (cfg->objects_per_lane != 1 && cfg->objects_per_lane != 2) || cfg->hbm_budget_bytes < (64ull << 20))
     return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_create: bad configuration");
   auto srv = std::make_unique<dpc_server>();
@@ -342,14 +353,21 @@ int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
   // a quarter of the budget (at most 1 GiB) is the in-flight request arena, split
   // over the streams; each stream has a device arena and a pinned host twin
   srv->arena_total = std::min<uint64_t>(cfg->hbm_budget_bytes / 4, 1ull << 30);
-  const uint64_t per = align_up(srv->arena_total / cfg->streams, 4096);
+  const uint32_t depth = cfg->batches_per_stream ? cfg->batches_per_stream : 2;
+  srv->cfg.batches_per_stream = depth;
+  const uint64_t per = align_up(srv->arena_total / (cfg->streams * depth), 4096);
   hipError_t e = hipMalloc(&srv->d_scene_table, sizeof(dpc_scene_desc) * cfg->max_scenes);
   if (e != hipSuccess) return hip_fail(e, "scene table");
   std::vector<dpc_scene_desc> zero(cfg->max_scenes, dpc_scene_desc{});
   e = hipMemcpy(srv->d_scene_table, zero.data(), sizeof(dpc_scene_desc) * cfg->max_scenes, hipMemcpyHostToDevice);
-  srv->slots.resize(cfg->streams);
-  for (auto& sl : srv->slots) {
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+  srv->streams.assign(cfg->streams, nullptr);
+  for (auto& st : srv->streams)
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+  srv->slots.resize(static_cast<size_t>(cfg->streams) * depth);
+  for (size_t k = 0; k < srv->slots.size(); ++k) {
+    StreamSlot& sl = srv->slots[k];
+    sl.stream_index = static_cast<int>(k % cfg->streams);
+    sl.stream = srv->streams[sl.stream_index];
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sl.d_arena), per);
     if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_arena), kHostArena, hipHostMallocDefault);
@@ -377,13 +395,15 @@ int dpc_server_destroy(dpc_server* s)
     s->cv_work.notify_all();
     s->worker.join();
   }
+  for (auto& st : s->streams)
+    if (st) (void)hipStreamSynchronize(st);
   for (auto& sl : s->slots) {
-    if (sl.stream) (void)hipStreamSynchronize(sl.stream);
     if (sl.d_arena) (void)hipFree(sl.d_arena);
     if (sl.h_arena) (void)hipHostFree(sl.h_arena);
     if (sl.done) (void)hipEventDestroy(sl.done);
-    if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
+  for (auto& st : s->streams)
+    if (st) (void)hipStreamDestroy(st);
   for (auto& sc : s->scenes) {
     if (!sc.live) continue;
     (void)hipFree(sc.box_a);
@@ -411,8 +431,8 @@ int dpc_server_add_scene(dpc_server* s, const float* lower_upper, uint64_t n, co
     id = 0;
     while (id < s->scenes.size() && s->scenes[id].live) ++id;
     if (id == s->scenes.size()) return fail(DPC_ERR_OVER_BUDGET, "dpc_server_add_scene: all scene slots are in use");
-    s->scenes[id].live = true;  // reserve the slot
-    s->scenes[id].busy = true;  // not schedulable until uploaded
+    s->scenes[id].live = true;    // reserve the slot
+    s->scenes[id].ready = false;  // not schedulable until uploaded
     s->scene_bytes += bytes;
   }
   Scene sc;
@@ -420,6 +440,7 @@ int dpc_server_add_scene(dpc_server* s, const float* lower_upper, uint64_t n, co
   sc.n_mat = n_mat;
   sc.bytes = bytes;
   sc.live = true;
+  sc.ready = true;
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&sc.box_a), n * 16);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc.box_b), n * 8);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sc.planes), n_mat * 64);
@@ -467,7 +488,7 @@ int dpc_server_remove_scene(dpc_server* s, uint32_t id)
   {
     std::unique_lock<std::mutex> lock(s->mu);
     if (id >= s->scenes.size() || !s->scenes[id].live) return fail(DPC_ERR_NOT_FOUND, "no such scene");
-    if (s->scenes[id].busy || s->scenes[id].pending)
+    if (!s->scenes[id].ready || s->scenes[id].inflight || s->scenes[id].pending)
       return fail(DPC_ERR_STATE, "dpc_server_remove_scene: the scene has requests in flight");
     sc = s->scenes[id];
     s->scenes[id] = Scene{};

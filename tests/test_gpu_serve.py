@@ -74,6 +74,11 @@ def test_serve_mix_equals_alone_and_oracle(orc):
     assert max(info.batch_requests for info, _ in mix) > 1
     spans = sorted({(info.batch, info.admit_us, info.done_us) for info, _ in mix})
     assert any(b[1] < a[2] and a[0] != b[0] for a, b in zip(spans, spans[1:])), "no two batches were in flight together"
+    # a scene's next request is admitted (on the same stream) before its previous one is done
+    by_scene = {}
+    for r, (info, _) in zip(reqs, mix):
+        by_scene.setdefault(r.scene, []).append(info)
+    assert any(b.admit_us < a.done_us for infos in by_scene.values() for a, b in zip(infos, infos[1:]))
 
     # the oracle, replaying every scene's deltas in submission order
     rep = Replay(orc, scenes)
@@ -99,8 +104,8 @@ def test_serve_budget_admission(orc):
     whole-scene deltas never share a batch, small requests do."""
     scenes = S.serving_scenes(seed=21, n_scenes=4, size_range=(50_000, 50_001))
     budget = 64 << 20
-    streams = 4
-    per_stream = ((budget // 4) // streams + 4095) // 4096 * 4096
+    streams, depth = 4, 1
+    per_slot = ((budget // 4) // (streams * depth) + 4095) // 4096 * 4096
     rng = np.random.default_rng(5)
     reqs = []
     for i in range(16):
@@ -114,7 +119,8 @@ def test_serve_budget_admission(orc):
             reqs.append(S.ServingRequest(s, vps, sc.perm, S.trs_matrices(rng, sc.n), 0.0))
         else:
             reqs.append(S.ServingRequest(s, vps, sc.perm[:1000], sc.pool[:1000], 0.0))
-    with Server(max_scenes=4, max_batch_requests=8, streams=streams, hbm_budget_bytes=budget) as srv:
+    with Server(max_scenes=4, max_batch_requests=8, streams=streams, hbm_budget_bytes=budget,
+                batches_per_stream=depth) as srv:
         ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scenes]
         pend = []
         for r in reqs:
@@ -123,7 +129,7 @@ def test_serve_budget_admission(orc):
         done = [(r, srv.wait(rid, 60_000.0), m) for r, rid, m in pend]
         st = srv.stats()
     assert all(info.status == 0 for _, info, _ in done)
-    assert 0 < st["peak_inflight_bytes"] <= per_stream
+    assert 0 < st["peak_inflight_bytes"] <= per_slot
     bigs_per_batch = {}
     for r, info, _ in done:
         if r.delta_idx.size == scenes[r.scene].n:
@@ -168,7 +174,7 @@ def test_serve_errors_and_scene_slots(orc):
         with pytest.raises(_lib.DpcError) as e:  # 65 frusta
             srv.submit(a, np.repeat(vp, 65, axis=0), np.zeros(65 * m.size, np.uint64))
         assert e.value.code == _lib.DPC_ERR_INVALID_ARGUMENT
-        with pytest.raises(_lib.DpcError) as e:  # a request larger than the in-flight arena (16 MiB)
+        with pytest.raises(_lib.DpcError) as e:  # a request larger than one batch arena (8 MiB here)
             k = 250_000
             srv.submit(a, vp, m, (np.arange(k) % sc.n).astype(np.uint32), np.zeros((k, 16), np.float32))
         assert e.value.code == _lib.DPC_ERR_OVER_BUDGET
