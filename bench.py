@@ -56,7 +56,8 @@ def parse():
                     help="launch only a 512 MiB device copy, the batched kernel and the flat kernel a few times "
                          "each (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes); prints no JSON")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
-    ap.add_argument("--serve-streams", type=int, default=4, help="batches in flight in the serving measurement")
+    ap.add_argument("--serve-streams", type=int, default=4, help="streams of the serving measurement")
+    ap.add_argument("--serve-depth", type=int, default=2, help="batches in flight per stream in the serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of each serving-rate run")
     ap.add_argument("--tune", default="", help="batch config k,block,waves_per_cu,max_frusta")
     return ap.parse_args()
@@ -149,7 +150,9 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     n_slots, slot_words = 192, 32 * ((max(sizes) + 63) // 64)
     out_pool = PinnedArray((n_slots, slot_words), np.uint64)
     streams = getattr(args, "serve_streams", 4)
-    srv = Server(max_scenes=16, max_batch_requests=16, streams=streams, hbm_budget_bytes=32 << 30)
+    depth = getattr(args, "serve_depth", 2)
+    srv = Server(max_scenes=16, max_batch_requests=16, streams=streams, hbm_budget_bytes=32 << 30,
+                 batches_per_stream=depth)
     ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scs]
     base_mats = [sc.matrices.reshape(-1, 16).copy() for sc in scs]
     runs = []
@@ -207,6 +210,8 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
         t_submitted = srv.now_us()
         th.join()
         lat = np.array([x.latency_us for x in results])
+        queue = np.array([x.admit_us - x.submit_us for x in results])
+        execu = np.array([x.done_us - x.admit_us for x in results])
         t_first = min(x.submit_us for x in results)
         t_last = max(x.done_us for x in results)
         tests = sum(sizes[r.scene] * r.vps.shape[0] for r in reqs)
@@ -220,11 +225,15 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
                          pcie_h2d_gbs=delta_bytes / span_s / 1e9, pcie_d2h_gbs=mask_bytes / span_s / 1e9,
                          offered_rate=rate, requests=n_req, achieved_submit_rate=n_req / ((t_submitted - start) * 1e-6),
                          p50_us=float(np.percentile(lat, 50)), p99_us=float(np.percentile(lat, 99)),
+                         p90_us=float(np.percentile(lat, 90)), max_us=float(lat.max()),
+                         queue_p50_us=float(np.percentile(queue, 50)), queue_p99_us=float(np.percentile(queue, 99)),
+                         exec_p50_us=float(np.percentile(execu, 50)), exec_p99_us=float(np.percentile(execu, 99)),
                          mean_us=float(lat.mean()), tests=tests,
                          tests_per_s=tests / ((t_last - t_first) * 1e-6), batches=batches,
                          mean_batch_requests=n_req / batches, submitter_blocked_us=blocked_us,
                          delta_matrices=int(sum(r.delta_idx.size for r in reqs))))
-        log(f"serving @ {rate}/s: p50 {runs[-1]['p50_us']:.0f} us, p99 {runs[-1]['p99_us']:.0f} us, "
+        log(f"serving @ {rate}/s: p50 {runs[-1]['p50_us']:.0f} us, p99 {runs[-1]['p99_us']:.0f} us "
+            f"(queue p99 {runs[-1]['queue_p99_us']:.0f}, exec p99 {runs[-1]['exec_p99_us']:.0f}), "
             f"{runs[-1]['tests_per_s'] / 1e9:.1f} G tests/s, {batches} batches, "
             f"submit rate {runs[-1]['achieved_submit_rate']:.0f}/s, PCIe in {runs[-1]['pcie_h2d_gbs']:.1f} GB/s, "
             f"out {runs[-1]['pcie_d2h_gbs']:.1f} GB/s, scheduler issuing {runs[-1]['scheduler_issue_frac']:.0%}")
@@ -257,7 +266,7 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     for p in pinned:
         p.free()
     return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check, pinned_copy=pcie,
-                server={"streams": streams, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
+                server={"streams": streams, "batches_per_stream": depth, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
                         "scene_bytes": st["scene_bytes"], "peak_inflight_bytes": st["peak_inflight_bytes"]})
 
 
@@ -441,6 +450,29 @@ def main():
     cpu_value = cpu_tests / cpu_time
     log(f"cpu oracle: {cpu_tests} tests in {cpu_time:.1f}s = {cpu_value / 1e6:.2f} M tests/s "
         f"({pairs_done} pairs, {time.perf_counter() - t_cpu0:.1f}s)")
+    # the same passes on every host core available (at most 16): the object
+    # range of each group split into whole mask words, one block per thread
+    threads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    mt_tests, mt_time, mt_pairs = 0, 0.0, 0
+    while mt_time < args.cpu_seconds / 2:
+        obb_cache = {}
+        for p in order:
+            f, g = int(bs.pair_frustum[p]), int(bs.pair_group[p])
+            a, n = int(bs.group_first[g]), int(bs.group_count[g])
+            t0 = time.perf_counter()
+            if g not in obb_cache:
+                obb_cache[g] = orc.obb_build_group_mt(l4[a:a + n], e4[a:a + n], np.arange(a, a + n), bs.matrices,
+                                                      threads)
+            orc.cull_obbs_mt(obb_cache[g], bs.vps[f], threads)
+            mt_time += time.perf_counter() - t0
+            mt_tests += n
+            mt_pairs += 1
+            if mt_time >= args.cpu_seconds / 2:
+                break
+        obb_cache.clear()
+    result["cpu_all_cores"] = dict(value=mt_tests / mt_time, unit="tests/s", cores=threads, tests=mt_tests,
+                                   pairs=mt_pairs, seconds=mt_time, cpu=cpu_model())
+    log(f"cpu oracle on {threads} threads: {mt_tests / mt_time / 1e6:.1f} M tests/s")
     scene.free()
     for d in (d_vps, masks, masks_pv):
         d.free()

@@ -301,7 +301,8 @@ int dpc_server_submit(dpc_server* server, uint32_t scene_id, const float* vps, u
                       const uint32_t* delta_idx, const float* delta_mats, uint32_t n_delta, uint64_t* masks_out,
                       uint64_t mask_capacity_words, uint64_t* request_id);
 /* Waits for a request (timeout_ms < 0: no limit) and fills info.  The request is
- * forgotten afterwards.  Returns the request's status; DPC_ERR_STATE on timeout. */
+ * forgotten afterwards.  Returns the request's status; DPC_ERR_STATE on timeout.
+ * A request whose batch failed to launch (status != DPC_OK) applied no delta. */
 int dpc_server_wait(dpc_server* server, uint64_t request_id, double timeout_ms, dpc_request_info* info);
 int dpc_server_stats_get(dpc_server* server, dpc_server_stats* stats);
 /* The server's clock (microseconds), to schedule arrivals against. */

@@ -10,6 +10,7 @@
 #include "oracle.h"
 
 #include <float.h>
+#include <pthread.h>
 #include <string.h>
 
 /* dp/math/Matmnt.h:1371-1379 */
@@ -295,4 +296,76 @@ int orc_invert_transpose(const float m[16], float out[16])
 #undef M
   memcpy(out, r, sizeof r);
   return 1;
+}
+
+/* ---- a static partition over host threads (CPU timing only) -----------------
+ * The same per-object functions as orc_obb_build_group and orc_cull_obbs, with
+ * the object range split into contiguous blocks of whole 64-bit mask words, one
+ * per thread.  Used to time the CPU path on every host core; results are
+ * identical to the single-threaded functions. */
+typedef struct {
+  size_t begin, end;
+  const float *lower4, *extent4;
+  const uint64_t* xform;
+  const void* matrices;
+  size_t stride;
+  const float* vp;
+  float* obbs;
+  uint32_t* mask;
+} orc_range;
+
+static void* obb_range(void* arg)
+{
+  const orc_range* r = (const orc_range*)arg;
+  for (size_t i = r->begin; i < r->end; ++i)
+    orc_obb_build(r->lower4 + 4 * i, r->extent4 + 4 * i, matrix_at(r->matrices, r->stride, r->xform[i]),
+                  r->obbs + 16 * i);
+  return NULL;
+}
+
+static void* cull_range(void* arg)
+{
+  const orc_range* r = (const orc_range*)arg;
+  for (size_t i = r->begin; i < r->end; ++i) set_bit(r->mask, i, orc_is_visible_obb(r->vp, r->obbs + 16 * i));
+  return NULL;
+}
+
+static void run_ranges(size_t n, int threads, orc_range proto, void* (*fn)(void*))
+{
+  enum { kMaxThreads = 256 };
+  if (threads < 1) threads = 1;
+  if (threads > kMaxThreads) threads = kMaxThreads;
+  pthread_t tid[kMaxThreads];
+  int created[kMaxThreads];
+  orc_range part[kMaxThreads];
+  const size_t words = (n + 63) / 64;
+  const size_t per = (words + (size_t)threads - 1) / (size_t)threads;
+  for (int t = 0; t < threads; ++t) {
+    part[t] = proto;
+    part[t].begin = (size_t)t * per * 64;
+    part[t].end = ((size_t)t + 1) * per * 64;
+    if (part[t].begin > n) part[t].begin = n;
+    if (part[t].end > n) part[t].end = n;
+  }
+  for (int t = 1; t < threads; ++t) {
+    created[t] = pthread_create(&tid[t], NULL, fn, &part[t]) == 0;
+    if (!created[t]) fn(&part[t]);  /* no thread: do its part here */
+  }
+  fn(&part[0]);
+  for (int t = 1; t < threads; ++t)
+    if (created[t]) pthread_join(tid[t], NULL);
+}
+
+void orc_obb_build_group_mt(size_t n, const float* lower4, const float* extent4, const uint64_t* xform,
+                            const void* matrices, size_t stride, float* obbs, int threads)
+{
+  orc_range r = {0, 0, lower4, extent4, xform, matrices, stride, NULL, obbs, NULL};
+  run_ranges(n, threads, r, obb_range);
+}
+
+void orc_cull_obbs_mt(size_t n, const float* obbs, const float vp[16], uint32_t* mask, int threads)
+{
+  memset(mask, 0, mask_words(n) * sizeof(uint32_t));
+  orc_range r = {0, 0, NULL, NULL, NULL, NULL, 0, vp, (float*)obbs, mask};
+  run_ranges(n, threads, r, cull_range);
 }

@@ -129,6 +129,13 @@ size_t orc_result_diff(size_t n, const uint32_t* old_mask, const uint32_t* new_m
  * (NaN included); returns 0 and leaves out untouched when det == 0. */
 int orc_invert_transpose(const float m[16], float out[16]);
 
+/* orc_obb_build_group and orc_cull_obbs over `threads` host threads (a static
+ * partition of the objects into whole 64-bit mask words); same results.  For
+ * timing the CPU path on every core. */
+void orc_obb_build_group_mt(size_t n, const float* lower4, const float* extent4, const uint64_t* xform,
+                            const void* matrices, size_t stride, float* obbs, int threads);
+void orc_cull_obbs_mt(size_t n, const float* obbs, const float vp[16], uint32_t* mask, int threads);
+
 #ifdef __cplusplus
 }
 #endif

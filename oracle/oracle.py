@@ -44,6 +44,8 @@ def load():
             "orc_tree_compute_f64": [SZ, P, P, P, P],
             "orc_result_diff": [SZ, P, P, P],
             "orc_invert_transpose": [P, P],
+            "orc_obb_build_group_mt": [SZ, P, P, P, P, SZ, P, C.c_int],
+            "orc_cull_obbs_mt": [SZ, P, P, P, C.c_int],
         }
         for name, args in sig.items():
             fn = getattr(lib, name)
@@ -238,3 +240,24 @@ def invert_transpose(ms):
     for i in range(m.shape[0]):
         ok[i] = fn(_p(m[i]), _p(out[i]))
     return ok, out.reshape(-1, 4, 4)
+
+
+def obb_build_group_mt(lower4, extent4, xform, matrices, threads: int) -> np.ndarray:
+    """obb_build_group over `threads` host threads (same result)."""
+    l4, e4 = _f32(lower4).reshape(-1, 4), _f32(extent4).reshape(-1, 4)
+    n = l4.shape[0]
+    x = np.ascontiguousarray(xform, np.uint64).reshape(n)
+    m = _f32(matrices).reshape(-1, 16)
+    out = np.empty((max(n, 1), 16), np.float32)
+    load().orc_obb_build_group_mt(n, _p(l4), _p(e4), _p(x), _p(m), 64, _p(out), int(threads))
+    return out[:n].reshape(n, 4, 4)
+
+
+def cull_obbs_mt(obbs, vp, threads: int) -> np.ndarray:
+    """cull_obbs over `threads` host threads (same result)."""
+    o = _f32(obbs).reshape(-1, 16)
+    n = o.shape[0]
+    vp = _f32(vp).reshape(16)
+    mask = np.zeros(max(mask_u32_words(n), 2), np.uint32)
+    load().orc_cull_obbs_mt(n, _p(o), _p(vp), _p(mask), int(threads))
+    return mask.view(np.uint64)[: (n + 63) // 64]

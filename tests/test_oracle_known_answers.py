@@ -182,3 +182,19 @@ def test_bbox_signed_zero_first_wins(orc):
     assert ab[0] == 0 and not np.signbit(ab[0])
     ba = orc.bbox_group(l4[::-1].copy(), e4, [1, 0], mats)
     assert ba[0] == 0 and np.signbit(ba[0])
+
+
+def test_threaded_drivers_match_single_thread(orc):
+    """The all-cores CPU timing path (a static partition into whole mask words)
+    gives the single-threaded results bit for bit, for ragged sizes too."""
+    from pipeline_amd import scenes
+    for n in (1, 63, 64, 65, 1000, 70_001):
+        s = scenes.flat_scene(n, 99 + n)
+        l4, e4 = scenes.extents(s.lower, s.upper)
+        x = np.arange(n)
+        o1 = orc.obb_build_group(l4, e4, x, s.matrices)
+        m1 = orc.cull_obbs(o1, s.vp)
+        for threads in (1, 2, 5, 16):
+            o = orc.obb_build_group_mt(l4, e4, x, s.matrices, threads)
+            assert np.array_equal(o.view(np.uint32), o1.view(np.uint32))
+            assert np.array_equal(orc.cull_obbs_mt(o1, s.vp, threads), m1)
