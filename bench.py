@@ -56,7 +56,7 @@ def parse():
                     help="launch only a 512 MiB device copy, the batched kernel and the flat kernel a few times "
                          "each (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes); prints no JSON")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
-    ap.add_argument("--serve-streams", type=int, default=4, help="streams of the serving measurement")
+    ap.add_argument("--serve-streams", type=int, default=8, help="streams of the serving measurement")
     ap.add_argument("--serve-depth", type=int, default=2, help="batches in flight per stream in the serving measurement")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of each serving-rate run")
     ap.add_argument("--tune", default="", help="batch config k,block,waves_per_cu,max_frusta")
@@ -149,12 +149,21 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     log(f"serving: pinned copies {pcie['h2d_gbs']:.1f} GB/s host->device, {pcie['d2h_gbs']:.1f} GB/s device->host")
     n_slots, slot_words = 192, 32 * ((max(sizes) + 63) // 64)
     out_pool = PinnedArray((n_slots, slot_words), np.uint64)
-    streams = getattr(args, "serve_streams", 4)
+    streams = getattr(args, "serve_streams", 8)
     depth = getattr(args, "serve_depth", 2)
     srv = Server(max_scenes=16, max_batch_requests=16, streams=streams, hbm_budget_bytes=32 << 30,
                  batches_per_stream=depth)
     ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scs]
     base_mats = [sc.matrices.reshape(-1, 16).copy() for sc in scs]
+    # warm-up (untimed): two requests per scene without deltas, so first launches,
+    # first copies on every stream and first touches are not in the measurement
+    warm = scenes.serving_requests(scs, seed=99, n_requests=32, rate_per_s=0.0)
+    for k in range(0, len(warm), n_slots):
+        ids_w = []
+        for j, r in enumerate(warm[k:k + n_slots]):
+            ids_w.append(srv.submit(ids[r.scene], r.vps, out_pool.array[j]))
+        for rid in ids_w:
+            srv.wait(rid)
     runs = []
     check = None
     switch = sys.getswitchinterval()
@@ -221,7 +230,12 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
         delta_bytes = sum(68 * r.delta_idx.size for r in reqs)
         mask_bytes = sum(8 * ((sizes[r.scene] + 63) // 64) * r.vps.shape[0] for r in reqs)
         span_s = (t_last - t_first) * 1e-6
-        runs.append(dict(_issue_total=st_now["issue_us"], scheduler_issue_frac=issue_us * 1e-6 / span_s,
+        worst = np.argsort(queue)[-10:][::-1]
+        worst_queue = [dict(at_ms=round((results[i].submit_us - start) / 1e3, 2), queue_us=round(float(queue[i])),
+                            exec_us=round(float(execu[i])), batch=results[i].batch,
+                            batch_requests=results[i].batch_requests, scene=reqs[i].scene,
+                            frusta=int(reqs[i].vps.shape[0]), delta=int(reqs[i].delta_idx.size)) for i in worst]
+        runs.append(dict(_issue_total=st_now["issue_us"], worst_queue=worst_queue, scheduler_issue_frac=issue_us * 1e-6 / span_s,
                          pcie_h2d_gbs=delta_bytes / span_s / 1e9, pcie_d2h_gbs=mask_bytes / span_s / 1e9,
                          offered_rate=rate, requests=n_req, achieved_submit_rate=n_req / ((t_submitted - start) * 1e-6),
                          p50_us=float(np.percentile(lat, 50)), p99_us=float(np.percentile(lat, 99)),
@@ -518,14 +532,23 @@ def main():
         d_parent = dev.DeviceArray.from_host(ts.parent_slot)
         d_dirty = dev.DeviceArray(((n_slots + 3) // 4) * 4, np.uint8)
         d_dirty.fill_u32(0)
-        off = (C.c_uint64 * len(ts.level_offsets))(*[int(x) for x in ts.level_offsets])
         objs = dev.upload_scene(ts.obj_lower, ts.obj_upper, np.eye(4, dtype=np.float32)[None], ts.obj_slot)
         tmask = dev.DeviceArray(dev.mask_words(objs.n_objects), np.uint64)
         tvp = dev.vp_arg(ts.vp)
 
+        # the launch schedule of this layout (tile runs for the deep levels)
+        tplan = C.c_void_p()
+        par_h = np.ascontiguousarray(ts.parent_slot, np.uint32)
+        off_h = np.ascontiguousarray(ts.level_offsets, np.uint64)
+        _lib.check(lib.dpc_tree_plan_create(par_h.ctypes.data, n_slots, off_h.ctypes.data, len(off_h) - 1,
+                                            C.byref(tplan)))
+        t_launches, t_runs, t_tiles = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _lib.check(lib.dpc_tree_plan_info(tplan, C.byref(t_launches), C.byref(t_runs), C.byref(t_tiles)))
+        log(f"tree plan: {t_launches.value} launches ({t_runs.value} tile runs, {t_tiles.value} tiles) "
+            f"for {len(off_h) - 1} levels")
+
         def issue_tree():
-            _lib.check(lib.dpc_tree_propagate(d_local.ptr, d_world.ptr, n_slots, d_parent.ptr, off,
-                                              len(ts.level_offsets) - 1, None, d_dirty.ptr, stream.handle))
+            _lib.check(lib.dpc_tree_propagate_plan(tplan, d_local.ptr, d_world.ptr, None, d_dirty.ptr, stream.handle))
             _lib.check(lib.dpc_cull(objs.box_a.ptr, objs.box_b.ptr, objs.xform.ptr, objs.n_objects, d_world.ptr,
                                     n_slots, tvp, tmask.ptr, stream.handle))
 
@@ -565,11 +588,13 @@ def main():
                               achieved_gbs=(prop_bytes + cull_bytes) / (k_mean_t * 1e-3) / 1e9,
                               tests_per_s=m_obj / (wall_t / max(args.steps // 4, 5)),
                               world_max_ulp_vs_oracle=max_ulp, world_max_rel_err_vs_fp64=max_rel,
-                              cull_mismatches=mism_t)
+                              cull_mismatches=mism_t,
+                              propagation_launches=t_launches.value, tile_runs=t_runs.value, tiles=t_tiles.value)
         log(f"tree: {result['tree']['us_per_step']:.1f} us/step (propagate + cull)")
         for d in (d_local, d_world, d_parent, d_dirty, tmask):
             d.free()
         objs.free()
+        lib.dpc_tree_plan_destroy(tplan)
 
         # ---------------- serving loop ----------------
         result["serving"] = bench_serving(args, lib, orc)

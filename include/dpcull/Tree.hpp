@@ -16,12 +16,14 @@
 //     world matrices to observers and clears the dirty state (Tree.cpp:133-166).
 //
 // Device layout: the level lists are compiled into "slots" (slot 0 = root, then
-// each level's entries in list order, then every index that is in no list), so
-// each level is a contiguous slot range and the kernels read and write whole
-// cache lines.  The compile runs only after structural changes; a level in which
-// an entry reads a matrix that another entry of the same level writes (possible
-// only after index reuse) is split so the sequential order of the reference is
-// kept.
+// each level's entries, then every index that is in no list), so each level is
+// a contiguous slot range and the kernels read and write whole cache lines.  The
+// compile runs only after structural changes; a level in which an entry reads a
+// matrix that another entry of the same level writes (possible only after index
+// reuse) is split into passes in list order, which keeps the sequential order
+// of the reference.  Inside a pass no entry reads what another writes, so the
+// pass is ordered by parent slot: the children of a slot range are then
+// contiguous, which the propagation plan's tile runs use.
 #pragma once
 
 #include <cstddef>

@@ -190,7 +190,8 @@ int dpc_cull_per_view(const dpc_batch_plan* plan, const float* h_vps, uint64_t* 
 /* ---- transform tree propagation ----------------------------------------------
  * dp::transform::Tree::compute (dp/transform/src/Tree.cpp:133-166) on a tree
  * laid out level-major: the entries of level l occupy slots
- * [h_level_offsets[l], h_level_offsets[l+1]) in the reference's list order,
+ * [h_level_offsets[l], h_level_offsets[l+1]) (any order inside a level whose
+ * entries do not read what other entries of it write),
  * and d_parent_slot[s] is the slot of entry s's parent.  For each level in
  * order, for each slot s of the level:
  *     if dirty_world[parent] || dirty_local[s]:  W[s] = L[s] * W[parent]; dirty_world[s] = 1
@@ -202,6 +203,24 @@ int dpc_cull_per_view(const dpc_batch_plan* plan, const float* h_vps, uint64_t* 
 int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const uint32_t* d_parent_slot,
                        const uint64_t* h_level_offsets, size_t n_levels, const uint8_t* d_dirty_local,
                        uint8_t* d_dirty_world, void* stream);
+/* The same propagation through a launch schedule built once per layout.
+ * dpc_tree_plan_create reads the layout on the host (h_parent_slot: ld entries;
+ * the level offsets as above) and keeps a device copy of the parent slots.
+ * Consecutive passes in which every entry's parent lies in the previous pass
+ * and which are ordered by parent slot become tile runs: one launch in which a
+ * workgroup carries a slot range of the first pass and its descendants through
+ * every pass of the run, with each pass's results in LDS for the next.  Small
+ * passes share a single-workgroup launch; any other pass is one launch.
+ * dpc_tree_propagate_plan has dpc_tree_propagate's semantics for that layout. */
+typedef struct dpc_tree_plan dpc_tree_plan;
+int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
+                         dpc_tree_plan** plan);
+int dpc_tree_plan_destroy(dpc_tree_plan* plan);
+/* launches per propagation, how many are tile runs, and their total tiles */
+int dpc_tree_plan_info(const dpc_tree_plan* plan, uint32_t* launches, uint32_t* tile_runs, uint64_t* tiles);
+int dpc_tree_propagate_plan(const dpc_tree_plan* plan, const float* d_local, float* d_world,
+                            const uint8_t* d_dirty_local, uint8_t* d_dirty_world, void* stream);
+
 /* Packs one byte per slot into BitArray words through a slot->index map:
  * bit d_index_of_slot[s] of d_bits = d_bytes[s] != 0 (d_bits must be cleared first;
  * it holds n_bits bits, and indices >= n_bits are skipped).

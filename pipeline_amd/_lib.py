@@ -122,6 +122,10 @@ _SIGNATURES = {
     "dpc_bbox_workspace": (SZ, [SZ]),
     "dpc_bbox": (I, [P, P, P, SZ, P, SZ, P, P, P]),
     "dpc_inverse_transpose": (I, [P, SZ, P, SZ, P, SZ, P, P]),
+    "dpc_tree_plan_create": (I, [P, SZ, P, SZ, C.POINTER(P)]),
+    "dpc_tree_plan_destroy": (I, [P]),
+    "dpc_tree_plan_info": (I, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U64)]),
+    "dpc_tree_propagate_plan": (I, [P, P, P, P, P, P]),
     "dpc_server_create": (I, [C.POINTER(ServerConfig), C.POINTER(P)]),
     "dpc_server_destroy": (I, [P]),
     "dpc_server_add_scene": (I, [P, P, U64, P, U64, P, C.POINTER(U32)]),

@@ -17,14 +17,14 @@ namespace dpc {
 namespace {
 
 constexpr unsigned kCullThreads = 256;  // persistent streaming form; plain form with an index array
-constexpr unsigned kPlainThreads = 512;  // plain form without an index array
+constexpr unsigned kPlainThreads = 1024;  // plain form without an index array
 
 // Plain form: one object per lane, one 64-object tile per wave, a grid that
 // covers all tiles.  The object's four matrix rows come from four row planes,
 // so every load of the wave is one contiguous 1 KiB (rows, box_a) or 512 B
 // (box_b) segment; the visibility bits leave as one ballot word per wave.
 // Without an index array every box and matrix is read once per launch, so
-// they are loaded as streaming data in 512-thread blocks; with one (matrices
+// they are loaded as streaming data in 1024-thread blocks; with one (matrices
 // shared by several objects, e.g. the leaves of a transform tree), plain cached
 // loads in 256-thread blocks measured faster.
 template <bool Indexed>

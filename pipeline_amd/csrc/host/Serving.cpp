@@ -317,10 +317,26 @@ void dpc_server::run()
         }
       }
     }
-    for (auto& slot : slots) {
-      if (!slot.busy && !pending.empty() && launch_batch(slot, lock)) {
-        progressed = true;
-        any_busy = true;
+    if (!pending.empty()) {
+      // free arenas of the least busy streams first: a request of a scene with
+      // nothing in flight then starts on an idle stream instead of queueing
+      // behind another batch (a scene bound to a busy stream can only go there)
+      std::vector<int> load(streams.size(), 0);
+      for (const auto& slot : slots) load[slot.stream_index] += slot.busy ? 1 : 0;
+      std::vector<bool> tried(slots.size(), false);
+      while (!pending.empty()) {
+        size_t best = slots.size();
+        for (size_t k = 0; k < slots.size(); ++k)
+          if (!slots[k].busy && !tried[k] &&
+              (best == slots.size() || load[slots[k].stream_index] < load[slots[best].stream_index]))
+            best = k;
+        if (best == slots.size()) break;
+        tried[best] = true;
+        if (launch_batch(slots[best], lock)) {
+          load[slots[best].stream_index]++;
+          progressed = true;
+          any_busy = true;
+        }
       }
     }
     if (stop && pending.empty() && !any_busy) break;

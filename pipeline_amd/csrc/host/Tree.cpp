@@ -150,9 +150,18 @@ TreeLayout::Compiled TreeLayout::compile() const
       touched.push_back(t);
       n_pass = std::max(n_pass, s + 1);
     }
+    std::vector<size_t> order;
     for (int ps = 0; ps < n_pass; ++ps) {
-      for (size_t e = 0; e < list.size(); ++e) {
-        if (pass[e] != ps) continue;
+      // Inside a pass no entry reads what another writes, so its order is free:
+      // ordering it by parent slot makes the children of a slot range
+      // contiguous (what tile runs of the propagation plan need).  Parents with
+      // no slot yet (unlisted indices, placed after every level) sort last.
+      order.clear();
+      for (size_t e = 0; e < list.size(); ++e)
+        if (pass[e] == ps) order.push_back(e);
+      auto key = [&](size_t e) { return c.slot_of_index[list[e].parent]; };
+      std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return key(x) < key(y); });
+      for (size_t e : order) {
         c.slot_of_index[list[e].transform] = static_cast<uint32_t>(c.index_of_slot.size());
         c.index_of_slot.push_back(list[e].transform);
         parent_index.push_back(list[e].parent);
@@ -185,8 +194,10 @@ struct Tree::Device {
   DevBuf<uint32_t> parent_slot, index_of_slot, slot_of_index, old_slot, idx, bits;
   DevBuf<uint8_t> dirty_local, dirty_world;
   DevBuf<float> delta;
+  dpc_tree_plan* plan = nullptr;  // launch schedule of the current layout
   ~Device()
   {
+    if (plan) dpc_tree_plan_destroy(plan);
     if (stream) dpc_stream_destroy(stream);
   }
 };
@@ -286,6 +297,11 @@ void Tree::recompile()
   check(dpc_fill_u32(reinterpret_cast<uint32_t*>(d.dirty_world.p), 0u, (cap + 4) / 4, d.stream), "clear dirty");
   check(dpc_stream_sync(d.stream), "recompile sync");
   d.ld = cap;
+  if (d.plan) dpc_tree_plan_destroy(d.plan);
+  d.plan = nullptr;
+  check(dpc_tree_plan_create(next.parent_slot.data(), cap, next.level_offsets.data(), next.level_offsets.size() - 1,
+                             &d.plan),
+        "dpc_tree_plan_create");
   m_compiled = std::move(next);
   m_structureDirty = false;
   m_itAll = true;  // slots moved: recompute every inverse transpose from the relaid-out worlds
@@ -318,10 +334,8 @@ void Tree::compute(Mat44f const&)
   std::vector<uint8_t> dirty_by_slot(cap, 0);
   for (size_t s = 0; s < cap; ++s) dirty_by_slot[s] = m_dirtyLocal[m_compiled.index_of_slot[s]];
   check(dpc_memcpy_h2d(d.dirty_local.p, dirty_by_slot.data(), cap, d.stream), "upload dirty");
-  const auto& off = m_compiled.level_offsets;
-  check(dpc_tree_propagate(d.local.p, d.world.p, d.ld, d.parent_slot.p, off.data(), off.size() - 1, d.dirty_local.p,
-                           d.dirty_world.p, d.stream),
-        "dpc_tree_propagate");
+  check(dpc_tree_propagate_plan(d.plan, d.local.p, d.world.p, d.dirty_local.p, d.dirty_world.p, d.stream),
+        "dpc_tree_propagate_plan");
   if (m_itEnabled) updateInverseTranspose();
   // EventWorldMatricesChanged: the dirty-world bytes as a bitset over indices.
   const size_t words32 = 2 * ((cap + 63) / 64);

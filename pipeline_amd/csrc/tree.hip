@@ -5,14 +5,16 @@
 //     if dirtyWorld[parent] || dirtyLocal[t]: world[t] = local[t] * world[parent]; dirtyWorld[t] = 1
 //
 // Device layout: local and world matrices in row planes, indexed by slot; the
-// slots of one level are contiguous, in the reference's list order, so the
-// loads of L[s] and the stores of W[s] are contiguous per wave.  Levels are
+// slots of one level are contiguous, so the loads of L[s] and the stores of
+// W[s] are contiguous per wave.  Levels are
 // dependent.  A large level is one launch.  A run of small levels (at most
 // kLdsNodes slots each) shares one single-workgroup launch: the workgroup keeps
 // the world matrices and dirty flags of the level it has just written in LDS,
 // where the next level reads its parents, and loads the next level's local
 // matrices and parent slots while it computes the current one, so each level
 // costs a barrier and LDS traffic instead of dependent trips to memory.
+#include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "cull_math.hpp"
@@ -211,6 +213,111 @@ __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const 
   }
 }
 
+// ---- tile runs -------------------------------------------------------------
+// A run of consecutive passes in which every entry's parent lies in the
+// previous pass and each pass is ordered by parent slot.  Then the children of
+// a contiguous slot range form a contiguous range of the next pass, and so on
+// down the run: a tile is a contiguous range of the run's first pass together
+// with those descendant ranges, at most kTileSlots slots per pass.  One
+// workgroup walks one tile pass by pass, keeping the previous pass's
+// recomputed rows and flags in LDS, where the next pass reads its parents.
+// Every parent read from memory is unchanged during the launch (a parent of
+// the run's first pass was written by an earlier launch; a parent the tile
+// did not recompute keeps its value), so the passes are ordered by LDS-only
+// barriers.
+constexpr unsigned kTileSlots = 1024;
+constexpr int kMaxTileLevels = 16;
+
+struct TileLevel {
+  uint32_t first;  // first slot of the tile in this pass
+  uint32_t count;  // slots of the tile in this pass (<= kTileSlots)
+};
+
+template <bool HasDirty>
+__device__ __forceinline__ void load_tile_slot(const float4* __restrict__ L, size_t ld,
+                                               const uint32_t* __restrict__ parent,
+                                               const uint8_t* __restrict__ dirty_local, TileLevel t, SlotIn& in)
+{
+  // unconditional and clamped (slot 0 exists), as load_level
+  const uint64_t c = threadIdx.x < t.count ? static_cast<uint64_t>(t.first) + threadIdx.x : 0;
+  in.p = parent[c];
+  in.dl = HasDirty ? dirty_local[c] : 1u;
+  in.l0 = L[c];
+  in.l1 = L[ld + c];
+  in.l2 = L[2 * ld + c];
+  in.l3 = L[3 * ld + c];
+}
+
+template <bool HasDirty>
+__device__ __forceinline__ void tile_level(const SlotIn& in, float4* W, size_t ld, uint8_t* dirty_world, TileLevel t,
+                                           TileLevel prev, bool first_pass, float4 (&lds_in)[4][kTileSlots],
+                                           uint8_t (&flag_in)[kTileSlots], float4 (&lds_out)[4][kTileSlots],
+                                           uint8_t (&flag_out)[kTileSlots])
+{
+  if (threadIdx.x >= t.count) return;
+  const uint64_t s = static_cast<uint64_t>(t.first) + threadIdx.x;
+  const uint64_t p = in.p;
+  const bool valid = p < ld;  // a parent outside the arrays is never read (as in propagate_slot)
+  // a parent of a later pass lies in the tile's previous range (by construction)
+  const bool staged = valid && !first_pass && p >= prev.first && p < static_cast<uint64_t>(prev.first) + prev.count;
+  const uint32_t i = staged ? static_cast<uint32_t>(p - prev.first) : 0u;
+  const uint32_t dp_lds = staged ? flag_in[i] : 0u;
+  bool dirty = false;
+  float4 p0, p1, p2, p3;
+  if (dp_lds) {
+    p0 = lds_in[0][i];
+    p1 = lds_in[1][i];
+    p2 = lds_in[2][i];
+    p3 = lds_in[3][i];
+    dirty = true;
+  } else if (valid && (!staged || in.dl)) {
+    uint32_t dp = 0;
+    load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
+    dirty = (in.dl | (staged ? 0u : dp)) != 0;
+  }
+  if (dirty) {
+    const float4 w0 = rowmat(in.l0, p0, p1, p2, p3);
+    const float4 w1 = rowmat(in.l1, p0, p1, p2, p3);
+    const float4 w2 = rowmat(in.l2, p0, p1, p2, p3);
+    const float4 w3 = rowmat(in.l3, p0, p1, p2, p3);
+    W[s] = w0;
+    W[ld + s] = w1;
+    W[2 * ld + s] = w2;
+    W[3 * ld + s] = w3;
+    dirty_world[s] = 1;
+    lds_out[0][threadIdx.x] = w0;
+    lds_out[1][threadIdx.x] = w1;
+    lds_out[2][threadIdx.x] = w2;
+    lds_out[3][threadIdx.x] = w3;
+  }
+  flag_out[threadIdx.x] = dirty ? 1 : 0;
+}
+
+template <bool HasDirty>
+__global__ __launch_bounds__(kTileSlots) void tree_tile_kernel(const float4* __restrict__ L, float4* W, size_t ld,
+                                                               const uint32_t* __restrict__ parent,
+                                                               const TileLevel* __restrict__ tiles, int n_levels,
+                                                               const uint8_t* __restrict__ dirty_local,
+                                                               uint8_t* dirty_world)
+{
+  __shared__ float4 rows[2][4][kTileSlots];
+  __shared__ uint8_t flags[2][kTileSlots];
+  const TileLevel* t = tiles + static_cast<size_t>(blockIdx.x) * n_levels;
+  const int last = n_levels - 1;
+  SlotIn A, B;
+  load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[0], A);
+  for (int l = 0; l < n_levels; l += 2) {
+    load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[l + 1 < n_levels ? l + 1 : last], B);
+    tile_level<HasDirty>(A, W, ld, dirty_world, t[l], t[l > 0 ? l - 1 : 0], l == 0, rows[1], flags[1], rows[0],
+                         flags[0]);
+    lds_barrier();
+    if (l + 1 >= n_levels) break;
+    load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[l + 2 < n_levels ? l + 2 : last], A);
+    tile_level<HasDirty>(B, W, ld, dirty_world, t[l + 1], t[l], false, rows[0], flags[0], rows[1], flags[1]);
+    lds_barrier();
+  }
+}
+
 // dp::math::invertTranspose (dp/math/Matmnt.h:1171-1201) of W[s]: 2x2 minors of
 // rows 0-1 (s*) and rows 2-3 (c*), the determinant, one correctly rounded
 // division, then each cofactor sum times 1/det, evaluated left to right as the
@@ -285,6 +392,116 @@ __global__ void bytes_to_bits_kernel(const uint8_t* __restrict__ bytes, const ui
 }  // namespace dpc
 
 using namespace dpc;
+
+// The launch schedule of one tree layout (dpc_tree_plan_create).
+struct dpc_tree_plan {
+  enum Kind { kSmall, kLevel, kTiles };
+  struct Step {
+    Kind kind;
+    LevelRun run;          // kSmall
+    uint64_t s0 = 0, s1 = 0;  // kLevel
+    int n_levels = 0;      // kTiles
+    uint32_t n_tiles = 0;
+    TileLevel* d_tiles = nullptr;
+  };
+  size_t ld = 0;
+  uint32_t* d_parent = nullptr;
+  std::vector<Step> steps;
+  uint64_t tiles = 0;
+  ~dpc_tree_plan()
+  {
+    for (auto& st : steps)
+      if (st.d_tiles) (void)hipFree(st.d_tiles);
+    if (d_parent) (void)hipFree(d_parent);
+  }
+};
+
+namespace {
+
+// Host side of the plan: which consecutive passes form tile runs, and their tiles.
+struct PlanBuilder {
+  const uint32_t* par;
+  const uint64_t* off;
+  size_t n;  // passes
+  std::vector<uint8_t> link;  // link[j]: every parent of pass j is in pass j-1 and pass j is ordered by parent
+
+  PlanBuilder(const uint32_t* parent, const uint64_t* offsets, size_t n_levels) : par(parent), off(offsets), n(n_levels)
+  {
+    link.assign(n, 0);
+    for (size_t j = 1; j < n; ++j) {
+      bool ok = off[j + 1] > off[j];
+      for (uint64_t s = off[j]; ok && s < off[j + 1]; ++s) {
+        ok = par[s] >= off[j - 1] && par[s] < off[j];
+        if (ok && s > off[j]) ok = par[s] >= par[s - 1];
+      }
+      link[j] = ok;
+    }
+  }
+
+  // first[v - off[j-1]] for v in [off[j-1], off[j]]: the first slot of pass j
+  // whose parent is >= v (pass j ordered by parent).
+  std::vector<uint32_t> first_child(size_t j) const
+  {
+    const uint64_t a = off[j - 1], b = off[j];
+    std::vector<uint32_t> f(b - a + 1);
+    uint64_t c = off[j];
+    for (uint64_t v = a; v <= b; ++v) {
+      while (c < off[j + 1] && par[c] < v) ++c;
+      f[v - a] = static_cast<uint32_t>(c);
+    }
+    return f;
+  }
+
+  // Descendant boundaries of the nodes of pass l: cb[k][x - off[l]] is the first
+  // slot of pass l + k below node x (x = off[l + 1] gives the end of the pass).
+  // Extends while every link holds and no single node has more than
+  // kTileSlots descendants in a pass; returns the number of passes covered.
+  size_t descend(size_t l, std::vector<std::vector<uint32_t>>& cb) const
+  {
+    cb.clear();
+    std::vector<uint32_t> cur(off[l + 1] - off[l] + 1);
+    for (size_t x = 0; x < cur.size(); ++x) cur[x] = static_cast<uint32_t>(off[l] + x);
+    cb.push_back(cur);
+    for (size_t j = l + 1; j < n && cb.size() < static_cast<size_t>(kMaxTileLevels); ++j) {
+      if (!link[j]) break;
+      const std::vector<uint32_t> f = first_child(j);
+      std::vector<uint32_t> next(cur.size());
+      uint32_t widest = 0;
+      for (size_t x = 0; x < cur.size(); ++x) {
+        next[x] = f[cur[x] - off[j - 1]];
+        if (x) widest = std::max(widest, next[x] - next[x - 1]);
+      }
+      if (widest > kTileSlots) break;
+      cb.push_back(next);
+      cur.swap(next);
+    }
+    return cb.size();
+  }
+
+  // Tiles of the run of passes [l, l + cb.size()): greedy contiguous ranges of
+  // pass l, each at most kTileSlots slots in every pass.
+  std::vector<TileLevel> tiles(size_t l, const std::vector<std::vector<uint32_t>>& cb) const
+  {
+    const size_t levels = cb.size();
+    const size_t nodes = off[l + 1] - off[l];
+    std::vector<TileLevel> out;
+    size_t x0 = 0;
+    while (x0 < nodes) {
+      size_t x1 = x0 + 1;
+      while (x1 < nodes) {
+        bool fits = true;
+        for (size_t k = 0; k < levels && fits; ++k) fits = cb[k][x1 + 1] - cb[k][x0] <= kTileSlots;
+        if (!fits) break;
+        ++x1;
+      }
+      for (size_t k = 0; k < levels; ++k) out.push_back(TileLevel{cb[k][x0], cb[k][x1] - cb[k][x0]});
+      x0 = x1;
+    }
+    return out;
+  }
+};
+
+}  // namespace
 
 extern "C" {
 
@@ -363,6 +580,136 @@ int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of
   hipLaunchKernelGGL(bytes_to_bits_kernel, dim3(static_cast<unsigned>((n_slots + 255) / 256)), dim3(256), 0,
                      as_stream(stream), d_bytes, d_index_of_slot, n_slots, d_bits, n_bits);
   return check_launch("bytes_to_bits_kernel");
+}
+
+int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
+                         dpc_tree_plan** out)
+{
+  if (!out) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: null plan pointer");
+  *out = nullptr;
+  if (!h_parent_slot || !h_level_offsets || n_levels == 0 || ld == 0 || ld > 0xFFFFFFFFull)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: bad arguments");
+  for (size_t l = 0; l < n_levels; ++l)
+    if (h_level_offsets[l] > h_level_offsets[l + 1])
+      return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: level offsets decrease at level %zu", l);
+  if (h_level_offsets[n_levels] > ld)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: levels end beyond ld");
+  auto plan = std::make_unique<dpc_tree_plan>();
+  plan->ld = ld;
+  DPC_HIP_TRY(hipMalloc(&plan->d_parent, ld * sizeof(uint32_t)), "dpc_tree_plan_create: parents");
+  DPC_HIP_TRY(hipMemcpy(plan->d_parent, h_parent_slot, ld * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "dpc_tree_plan_create: parents");
+  PlanBuilder pb(h_parent_slot, h_level_offsets, n_levels);
+  const uint64_t* off = h_level_offsets;
+  std::vector<std::vector<uint32_t>> cb, cb_next;
+  size_t l = 0;
+  while (l < n_levels) {
+    const uint64_t size = off[l + 1] - off[l];
+    if (size == 0) {
+      ++l;
+      continue;
+    }
+    if (size <= kLdsNodes) {  // consecutive small passes: one single-workgroup run
+      dpc_tree_plan::Step st{};
+      st.kind = dpc_tree_plan::kSmall;
+      st.run.n = 0;
+      st.run.off[0] = off[l];
+      while (l < n_levels && st.run.n < kMaxLevelsPerSmall && off[l + 1] - st.run.off[0] <= kLdsNodes) {
+        st.run.off[st.run.n + 1] = off[l + 1];
+        ++st.run.n;
+        ++l;
+      }
+      plan->steps.push_back(st);
+      continue;
+    }
+    // a tile run, unless starting at the next pass reaches deeper
+    const size_t reach = pb.descend(l, cb);
+    bool tile = reach >= 2;
+    if (tile && l + reach < n_levels && l + 1 < n_levels && off[l + 2] - off[l + 1] > kLdsNodes) {
+      const size_t reach_next = pb.descend(l + 1, cb_next);
+      if (l + 1 + reach_next > l + reach) tile = false;
+    }
+    if (tile) {
+      pb.descend(l, cb);  // cb may have been reused above
+      const std::vector<TileLevel> t = pb.tiles(l, cb);
+      dpc_tree_plan::Step st{};
+      st.kind = dpc_tree_plan::kTiles;
+      st.n_levels = static_cast<int>(cb.size());
+      st.n_tiles = static_cast<uint32_t>(t.size() / cb.size());
+      DPC_HIP_TRY(hipMalloc(&st.d_tiles, t.size() * sizeof(TileLevel)), "dpc_tree_plan_create: tiles");
+      plan->steps.push_back(st);
+      DPC_HIP_TRY(hipMemcpy(st.d_tiles, t.data(), t.size() * sizeof(TileLevel), hipMemcpyHostToDevice),
+                  "dpc_tree_plan_create: tiles");
+      plan->tiles += st.n_tiles;
+      l += cb.size();
+      continue;
+    }
+    dpc_tree_plan::Step st{};
+    st.kind = dpc_tree_plan::kLevel;
+    st.s0 = off[l];
+    st.s1 = off[l + 1];
+    plan->steps.push_back(st);
+    ++l;
+  }
+  *out = plan.release();
+  return DPC_OK;
+}
+
+int dpc_tree_plan_destroy(dpc_tree_plan* plan)
+{
+  delete plan;
+  return DPC_OK;
+}
+
+int dpc_tree_plan_info(const dpc_tree_plan* plan, uint32_t* launches, uint32_t* tile_runs, uint64_t* tiles)
+{
+  if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_info: null plan");
+  uint32_t runs = 0;
+  for (const auto& st : plan->steps) runs += st.kind == dpc_tree_plan::kTiles ? 1 : 0;
+  if (launches) *launches = static_cast<uint32_t>(plan->steps.size());
+  if (tile_runs) *tile_runs = runs;
+  if (tiles) *tiles = plan->tiles;
+  return DPC_OK;
+}
+
+int dpc_tree_propagate_plan(const dpc_tree_plan* plan, const float* d_local, float* d_world,
+                            const uint8_t* d_dirty_local, uint8_t* d_dirty_world, void* stream)
+{
+  if (!plan || !d_local || !d_world || !d_dirty_world)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_propagate_plan: null argument");
+  if (!aligned(d_local, 16) || !aligned(d_world, 16))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_propagate_plan: misaligned matrix planes");
+  const auto* L = reinterpret_cast<const float4*>(d_local);
+  auto* W = reinterpret_cast<float4*>(d_world);
+  const size_t ld = plan->ld;
+  hipStream_t s = as_stream(stream);
+  for (const auto& st : plan->steps) {
+    switch (st.kind) {
+      case dpc_tree_plan::kSmall:
+        if (d_dirty_local)
+          hipLaunchKernelGGL(tree_small_levels_kernel<true>, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld,
+                             plan->d_parent, st.run, d_dirty_local, d_dirty_world);
+        else
+          hipLaunchKernelGGL(tree_small_levels_kernel<false>, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld,
+                             plan->d_parent, st.run, d_dirty_local, d_dirty_world);
+        break;
+      case dpc_tree_plan::kLevel:
+        hipLaunchKernelGGL(tree_level_kernel, dim3(static_cast<unsigned>((st.s1 - st.s0 + kTreeThreads - 1) / kTreeThreads)),
+                           dim3(kTreeThreads), 0, s, L, W, ld, plan->d_parent, st.s0, st.s1, d_dirty_local,
+                           d_dirty_world);
+        break;
+      case dpc_tree_plan::kTiles:
+        if (d_dirty_local)
+          hipLaunchKernelGGL(tree_tile_kernel<true>, dim3(st.n_tiles), dim3(kTileSlots), 0, s, L, W, ld,
+                             plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+        else
+          hipLaunchKernelGGL(tree_tile_kernel<false>, dim3(st.n_tiles), dim3(kTileSlots), 0, s, L, W, ld,
+                             plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+        break;
+    }
+    if (int rc = check_launch("dpc_tree_propagate_plan")) return rc;
+  }
+  return DPC_OK;
 }
 
 }  // extern "C"

@@ -35,16 +35,32 @@ class DeviceTree:
         self.dirty_world.fill_u32(0)
         self.offsets = np.ascontiguousarray(ts.level_offsets, np.uint64)
 
-    def propagate(self, dirty_local=None, stream=None):
+    def plan(self):
+        """dpc_tree_plan_create on this layout: (handle, launches, tile runs, tiles)."""
+        import ctypes as C
+        lib = _lib.load()
+        h = C.c_void_p()
+        par = np.ascontiguousarray(self.ts.parent_slot, np.uint32)
+        _lib.check(lib.dpc_tree_plan_create(par.ctypes.data, self.n, self.offsets.ctypes.data, self.offsets.size - 1,
+                                            C.byref(h)))
+        launches, runs, tiles = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _lib.check(lib.dpc_tree_plan_info(h, C.byref(launches), C.byref(runs), C.byref(tiles)))
+        return h, launches.value, runs.value, tiles.value
+
+    def propagate(self, dirty_local=None, stream=None, plan=None):
         import ctypes as C
         lib = _lib.load()
         d_dl = None
         if dirty_local is not None:
             d_dl = dev.DeviceArray.from_host(dirty_local)
-        off = (C.c_uint64 * self.offsets.size)(*self.offsets.tolist())
-        _lib.check(lib.dpc_tree_propagate(self.local.ptr, self.world.ptr, self.n, self.parent.ptr, off,
-                                          self.offsets.size - 1, d_dl.ptr if d_dl else None, self.dirty_world.ptr,
-                                          stream))
+        if plan is not None:
+            _lib.check(lib.dpc_tree_propagate_plan(plan, self.local.ptr, self.world.ptr, d_dl.ptr if d_dl else None,
+                                                   self.dirty_world.ptr, stream))
+        else:
+            off = (C.c_uint64 * self.offsets.size)(*self.offsets.tolist())
+            _lib.check(lib.dpc_tree_propagate(self.local.ptr, self.world.ptr, self.n, self.parent.ptr, off,
+                                              self.offsets.size - 1, d_dl.ptr if d_dl else None,
+                                              self.dirty_world.ptr, stream))
         _lib.check(lib.dpc_stream_sync(stream))
         if d_dl:
             d_dl.free()
@@ -75,10 +91,14 @@ def oracle_world(orc, ts):
     # were written by an earlier launch
     (6_500, 9, {"roots": 3_000, "p": 0.99}),
 ])
-def test_propagate_matches_oracle(orc, n_nodes, seed, kw):
+@pytest.mark.parametrize("use_plan", [False, True], ids=["levels", "plan"])
+def test_propagate_matches_oracle(orc, n_nodes, seed, kw, use_plan):
     ts = scenes.tree_scene(n_nodes, seed, **kw)
     t = DeviceTree(ts)
-    t.propagate()
+    plan = t.plan()[0] if use_plan else None
+    t.propagate(plan=plan)
+    if plan is not None:
+        _lib.load().dpc_tree_plan_destroy(plan)
     got = t.world_by_slot()
     want, event = oracle_world(orc, ts)
     want_by_slot = want[ts.index_of_slot]
@@ -171,4 +191,49 @@ def test_dirty_propagation(orc):
     assert np.array_equal(got_bits, event)
     for d in (d_idx, d_new, bits, d_map):
         d.free()
+    t.free()
+
+
+def test_plan_tiles_config2_tree_and_dirty_frames(orc):
+    """The 1,000,000-node tree through the plan: its deep levels form a tile run;
+    a full compute and then two partial ones (a few hundred dirty locals) give
+    the oracle's world matrices and events bit for bit."""
+    ts = scenes.tree_scene(1_000_000, 2)
+    t = DeviceTree(ts)
+    plan, launches, runs, tiles = t.plan()
+    print(f"plan: {launches} launches, {runs} tile runs, {tiles} tiles for {len(ts.level_offsets) - 1} passes")
+    assert runs >= 1 and launches < len(ts.level_offsets) - 1
+    offsets, entries = ts.reference_levels()
+    local = np.ascontiguousarray(ts.local_by_index().reshape(-1, 16))
+    world = np.zeros_like(local)
+    world[0] = np.eye(4, dtype=F).reshape(16)
+    rng = np.random.default_rng(11)
+    import ctypes as C
+    lib = _lib.load()
+    for frame in range(3):
+        if frame == 0:
+            dl_slots = np.ones(ts.n_slots, np.uint8)
+            t.propagate(plan=plan)
+        else:
+            slots = rng.choice(np.arange(1, ts.n_slots), 300, replace=False)
+            new = scenes.trs_matrices(rng, 300, scale=(0.8, 1.25), trans=(-10, 10))
+            d_idx = dev.DeviceArray.from_host(slots.astype(np.uint32))
+            d_new = dev.DeviceArray.from_host(new.reshape(-1, 16))
+            _lib.check(lib.dpc_matrices_scatter(t.local.ptr, t.n, d_idx.ptr, d_new.ptr, 300, None))
+            d_idx.free()
+            d_new.free()
+            dl_slots = np.zeros(ts.n_slots, np.uint8)
+            dl_slots[slots] = 1
+            local[ts.index_of_slot[slots]] = new.reshape(-1, 16)
+            t.propagate(dirty_local=dl_slots, plan=plan)
+        dl = np.zeros(ts.n_slots, np.uint8)
+        dl[ts.index_of_slot] = dl_slots
+        event = orc.tree_compute(offsets, entries, local, world, dl, np.zeros(ts.n_slots, np.uint8))
+        got = t.world_by_slot()
+        want = world.reshape(-1, 4, 4)[ts.index_of_slot]
+        assert np.array_equal(got[1:].view(np.uint32), want[1:].view(np.uint32)), f"frame {frame}"
+        dw = t.dirty_world.download()[: ts.n_slots]
+        assert np.array_equal(dw[1:], event[ts.index_of_slot][1:]), f"frame {frame}"
+        t.dirty_world.fill_u32(0)
+    lib.dpc_tree_plan_destroy(plan)
     t.free()
