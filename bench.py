@@ -235,7 +235,8 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
                             exec_us=round(float(execu[i])), batch=results[i].batch,
                             batch_requests=results[i].batch_requests, scene=reqs[i].scene,
                             frusta=int(reqs[i].vps.shape[0]), delta=int(reqs[i].delta_idx.size)) for i in worst]
-        runs.append(dict(_issue_total=st_now["issue_us"], worst_queue=worst_queue, scheduler_issue_frac=issue_us * 1e-6 / span_s,
+        runs.append(dict(_issue_total=st_now["issue_us"], worst_queue=worst_queue,
+                         scheduler_issue_max_us_so_far=st_now["issue_max_us"], scheduler_issue_frac=issue_us * 1e-6 / span_s,
                          pcie_h2d_gbs=delta_bytes / span_s / 1e9, pcie_d2h_gbs=mask_bytes / span_s / 1e9,
                          offered_rate=rate, requests=n_req, achieved_submit_rate=n_req / ((t_submitted - start) * 1e-6),
                          p50_us=float(np.percentile(lat, 50)), p99_us=float(np.percentile(lat, 99)),
@@ -540,7 +541,7 @@ def main():
         tplan = C.c_void_p()
         par_h = np.ascontiguousarray(ts.parent_slot, np.uint32)
         off_h = np.ascontiguousarray(ts.level_offsets, np.uint64)
-        _lib.check(lib.dpc_tree_plan_create(par_h.ctypes.data, n_slots, off_h.ctypes.data, len(off_h) - 1,
+        _lib.check(lib.dpc_tree_plan_create(par_h.ctypes.data, n_slots, off_h.ctypes.data, len(off_h) - 1, 0,
                                             C.byref(tplan)))
         t_launches, t_runs, t_tiles = C.c_uint32(), C.c_uint32(), C.c_uint64()
         _lib.check(lib.dpc_tree_plan_info(tplan, C.byref(t_launches), C.byref(t_runs), C.byref(t_tiles)))

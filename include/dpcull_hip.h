@@ -214,7 +214,7 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
  * dpc_tree_propagate_plan has dpc_tree_propagate's semantics for that layout. */
 typedef struct dpc_tree_plan dpc_tree_plan;
 int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
-                         dpc_tree_plan** plan);
+                         uint32_t tile_slots, dpc_tree_plan** plan);  /* tile_slots: 512, 1024 or 0 (default) */
 int dpc_tree_plan_destroy(dpc_tree_plan* plan);
 /* launches per propagation, how many are tile runs, and their total tiles */
 int dpc_tree_plan_info(const dpc_tree_plan* plan, uint32_t* launches, uint32_t* tile_runs, uint64_t* tiles);
@@ -300,6 +300,7 @@ typedef struct dpc_server_stats {
   uint64_t peak_inflight_bytes;
   double busy_us;            /* sum over batches of launch-to-completion time */
   double issue_us;           /* scheduler-thread time spent issuing batches (copies, launches) */
+  double issue_max_us;       /* the longest single batch issue */
 } dpc_server_stats;
 
 int dpc_server_create(const dpc_server_config* cfg, dpc_server** server);

@@ -54,7 +54,8 @@ class RequestInfo(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_uint64), ("batches", C.c_uint64), ("tests", C.c_uint64),
                 ("delta_matrices", C.c_uint64), ("scene_bytes", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
-                ("busy_us", C.c_double), ("issue_us", C.c_double)]
+                ("busy_us", C.c_double), ("issue_us", C.c_double),
+                ("issue_max_us", C.c_double)]
 
 
 class CullConfig(C.Structure):
@@ -122,7 +123,7 @@ _SIGNATURES = {
     "dpc_bbox_workspace": (SZ, [SZ]),
     "dpc_bbox": (I, [P, P, P, SZ, P, SZ, P, P, P]),
     "dpc_inverse_transpose": (I, [P, SZ, P, SZ, P, SZ, P, P]),
-    "dpc_tree_plan_create": (I, [P, SZ, P, SZ, C.POINTER(P)]),
+    "dpc_tree_plan_create": (I, [P, SZ, P, SZ, U32, C.POINTER(P)]),
     "dpc_tree_plan_destroy": (I, [P]),
     "dpc_tree_plan_info": (I, [P, C.POINTER(U32), C.POINTER(U32), C.POINTER(U64)]),
     "dpc_tree_propagate_plan": (I, [P, P, P, P, P, P]),

@@ -264,6 +264,7 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   const double t_issue1 = now();
   lock.lock();
   stats.issue_us += t_issue1 - t_issue0;
+  stats.issue_max_us = std::max(stats.issue_max_us, t_issue1 - t_issue0);
   slot.reqs = pick;
   slot.busy = true;
   slot.t_launch = t_admit;

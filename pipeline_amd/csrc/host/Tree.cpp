@@ -299,7 +299,7 @@ void Tree::recompile()
   d.ld = cap;
   if (d.plan) dpc_tree_plan_destroy(d.plan);
   d.plan = nullptr;
-  check(dpc_tree_plan_create(next.parent_slot.data(), cap, next.level_offsets.data(), next.level_offsets.size() - 1,
+  check(dpc_tree_plan_create(next.parent_slot.data(), cap, next.level_offsets.data(), next.level_offsets.size() - 1, 0,
                              &d.plan),
         "dpc_tree_plan_create");
   m_compiled = std::move(next);

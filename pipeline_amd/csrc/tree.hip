@@ -225,7 +225,8 @@ __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const 
 // the run's first pass was written by an earlier launch; a parent the tile
 // did not recompute keeps its value), so the passes are ordered by LDS-only
 // barriers.
-constexpr unsigned kTileSlots = 1024;
+constexpr unsigned kTileSlots = 1024;  // the widest tile (one slot per thread)
+constexpr unsigned kDefaultTileSlots = 1024;
 constexpr int kMaxTileLevels = 16;
 
 struct TileLevel {
@@ -248,11 +249,11 @@ __device__ __forceinline__ void load_tile_slot(const float4* __restrict__ L, siz
   in.l3 = L[3 * ld + c];
 }
 
-template <bool HasDirty>
+template <bool HasDirty, unsigned Slots>
 __device__ __forceinline__ void tile_level(const SlotIn& in, float4* W, size_t ld, uint8_t* dirty_world, TileLevel t,
-                                           TileLevel prev, bool first_pass, float4 (&lds_in)[4][kTileSlots],
-                                           uint8_t (&flag_in)[kTileSlots], float4 (&lds_out)[4][kTileSlots],
-                                           uint8_t (&flag_out)[kTileSlots])
+                                           TileLevel prev, bool first_pass, float4 (&lds_in)[4][Slots],
+                                           uint8_t (&flag_in)[Slots], float4 (&lds_out)[4][Slots],
+                                           uint8_t (&flag_out)[Slots])
 {
   if (threadIdx.x >= t.count) return;
   const uint64_t s = static_cast<uint64_t>(t.first) + threadIdx.x;
@@ -293,27 +294,27 @@ __device__ __forceinline__ void tile_level(const SlotIn& in, float4* W, size_t l
   flag_out[threadIdx.x] = dirty ? 1 : 0;
 }
 
-template <bool HasDirty>
-__global__ __launch_bounds__(kTileSlots) void tree_tile_kernel(const float4* __restrict__ L, float4* W, size_t ld,
-                                                               const uint32_t* __restrict__ parent,
-                                                               const TileLevel* __restrict__ tiles, int n_levels,
-                                                               const uint8_t* __restrict__ dirty_local,
-                                                               uint8_t* dirty_world)
+template <bool HasDirty, unsigned Slots>
+__global__ __launch_bounds__(Slots) void tree_tile_kernel(const float4* __restrict__ L, float4* W, size_t ld,
+                                                          const uint32_t* __restrict__ parent,
+                                                          const TileLevel* __restrict__ tiles, int n_levels,
+                                                          const uint8_t* __restrict__ dirty_local,
+                                                          uint8_t* dirty_world)
 {
-  __shared__ float4 rows[2][4][kTileSlots];
-  __shared__ uint8_t flags[2][kTileSlots];
+  __shared__ float4 rows[2][4][Slots];
+  __shared__ uint8_t flags[2][Slots];
   const TileLevel* t = tiles + static_cast<size_t>(blockIdx.x) * n_levels;
   const int last = n_levels - 1;
   SlotIn A, B;
   load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[0], A);
   for (int l = 0; l < n_levels; l += 2) {
     load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[l + 1 < n_levels ? l + 1 : last], B);
-    tile_level<HasDirty>(A, W, ld, dirty_world, t[l], t[l > 0 ? l - 1 : 0], l == 0, rows[1], flags[1], rows[0],
-                         flags[0]);
+    tile_level<HasDirty, Slots>(A, W, ld, dirty_world, t[l], t[l > 0 ? l - 1 : 0], l == 0, rows[1], flags[1],
+                                rows[0], flags[0]);
     lds_barrier();
     if (l + 1 >= n_levels) break;
     load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[l + 2 < n_levels ? l + 2 : last], A);
-    tile_level<HasDirty>(B, W, ld, dirty_world, t[l + 1], t[l], false, rows[0], flags[0], rows[1], flags[1]);
+    tile_level<HasDirty, Slots>(B, W, ld, dirty_world, t[l + 1], t[l], false, rows[0], flags[0], rows[1], flags[1]);
     lds_barrier();
   }
 }
@@ -405,6 +406,7 @@ struct dpc_tree_plan {
     TileLevel* d_tiles = nullptr;
   };
   size_t ld = 0;
+  uint32_t tile_slots = kTileSlots;
   uint32_t* d_parent = nullptr;
   std::vector<Step> steps;
   uint64_t tiles = 0;
@@ -423,6 +425,7 @@ struct PlanBuilder {
   const uint32_t* par;
   const uint64_t* off;
   size_t n;  // passes
+  uint32_t width = kTileSlots;  // slots of a tile per pass
   std::vector<uint8_t> link;  // link[j]: every parent of pass j is in pass j-1 and pass j is ordered by parent
 
   PlanBuilder(const uint32_t* parent, const uint64_t* offsets, size_t n_levels) : par(parent), off(offsets), n(n_levels)
@@ -454,8 +457,8 @@ struct PlanBuilder {
 
   // Descendant boundaries of the nodes of pass l: cb[k][x - off[l]] is the first
   // slot of pass l + k below node x (x = off[l + 1] gives the end of the pass).
-  // Extends while every link holds and no single node has more than
-  // kTileSlots descendants in a pass; returns the number of passes covered.
+  // Extends while every link holds and no single node has more than `width`
+  // descendants in a pass; returns the number of passes covered.
   size_t descend(size_t l, std::vector<std::vector<uint32_t>>& cb) const
   {
     cb.clear();
@@ -471,7 +474,7 @@ struct PlanBuilder {
         next[x] = f[cur[x] - off[j - 1]];
         if (x) widest = std::max(widest, next[x] - next[x - 1]);
       }
-      if (widest > kTileSlots) break;
+      if (widest > width) break;
       cb.push_back(next);
       cur.swap(next);
     }
@@ -479,7 +482,7 @@ struct PlanBuilder {
   }
 
   // Tiles of the run of passes [l, l + cb.size()): greedy contiguous ranges of
-  // pass l, each at most kTileSlots slots in every pass.
+  // pass l, each at most `width` slots in every pass.
   std::vector<TileLevel> tiles(size_t l, const std::vector<std::vector<uint32_t>>& cb) const
   {
     const size_t levels = cb.size();
@@ -490,7 +493,7 @@ struct PlanBuilder {
       size_t x1 = x0 + 1;
       while (x1 < nodes) {
         bool fits = true;
-        for (size_t k = 0; k < levels && fits; ++k) fits = cb[k][x1 + 1] - cb[k][x0] <= kTileSlots;
+        for (size_t k = 0; k < levels && fits; ++k) fits = cb[k][x1 + 1] - cb[k][x0] <= width;
         if (!fits) break;
         ++x1;
       }
@@ -583,7 +586,7 @@ int dpc_bytes_to_bits_indexed(const uint8_t* d_bytes, const uint32_t* d_index_of
 }
 
 int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
-                         dpc_tree_plan** out)
+                         uint32_t tile_slots, dpc_tree_plan** out)
 {
   if (!out) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: null plan pointer");
   *out = nullptr;
@@ -594,12 +597,16 @@ int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_
       return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: level offsets decrease at level %zu", l);
   if (h_level_offsets[n_levels] > ld)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: levels end beyond ld");
+  if (tile_slots != 0 && tile_slots != 512 && tile_slots != 1024)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: tile_slots must be 0, 512 or 1024");
   auto plan = std::make_unique<dpc_tree_plan>();
   plan->ld = ld;
+  plan->tile_slots = tile_slots ? tile_slots : kDefaultTileSlots;
   DPC_HIP_TRY(hipMalloc(&plan->d_parent, ld * sizeof(uint32_t)), "dpc_tree_plan_create: parents");
   DPC_HIP_TRY(hipMemcpy(plan->d_parent, h_parent_slot, ld * sizeof(uint32_t), hipMemcpyHostToDevice),
               "dpc_tree_plan_create: parents");
   PlanBuilder pb(h_parent_slot, h_level_offsets, n_levels);
+  pb.width = plan->tile_slots;
   const uint64_t* off = h_level_offsets;
   std::vector<std::vector<uint32_t>> cb, cb_next;
   size_t l = 0;
@@ -699,12 +706,21 @@ int dpc_tree_propagate_plan(const dpc_tree_plan* plan, const float* d_local, flo
                            d_dirty_world);
         break;
       case dpc_tree_plan::kTiles:
-        if (d_dirty_local)
-          hipLaunchKernelGGL(tree_tile_kernel<true>, dim3(st.n_tiles), dim3(kTileSlots), 0, s, L, W, ld,
-                             plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
-        else
-          hipLaunchKernelGGL(tree_tile_kernel<false>, dim3(st.n_tiles), dim3(kTileSlots), 0, s, L, W, ld,
-                             plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+        if (plan->tile_slots == 512) {
+          if (d_dirty_local)
+            hipLaunchKernelGGL((tree_tile_kernel<true, 512>), dim3(st.n_tiles), dim3(512), 0, s, L, W, ld,
+                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+          else
+            hipLaunchKernelGGL((tree_tile_kernel<false, 512>), dim3(st.n_tiles), dim3(512), 0, s, L, W, ld,
+                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+        } else {
+          if (d_dirty_local)
+            hipLaunchKernelGGL((tree_tile_kernel<true, 1024>), dim3(st.n_tiles), dim3(1024), 0, s, L, W, ld,
+                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+          else
+            hipLaunchKernelGGL((tree_tile_kernel<false, 1024>), dim3(st.n_tiles), dim3(1024), 0, s, L, W, ld,
+                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
+        }
         break;
     }
     if (int rc = check_launch("dpc_tree_propagate_plan")) return rc;

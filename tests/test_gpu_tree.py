@@ -35,14 +35,14 @@ class DeviceTree:
         self.dirty_world.fill_u32(0)
         self.offsets = np.ascontiguousarray(ts.level_offsets, np.uint64)
 
-    def plan(self):
+    def plan(self, tile_slots=0):
         """dpc_tree_plan_create on this layout: (handle, launches, tile runs, tiles)."""
         import ctypes as C
         lib = _lib.load()
         h = C.c_void_p()
         par = np.ascontiguousarray(self.ts.parent_slot, np.uint32)
         _lib.check(lib.dpc_tree_plan_create(par.ctypes.data, self.n, self.offsets.ctypes.data, self.offsets.size - 1,
-                                            C.byref(h)))
+                                            tile_slots, C.byref(h)))
         launches, runs, tiles = C.c_uint32(), C.c_uint32(), C.c_uint64()
         _lib.check(lib.dpc_tree_plan_info(h, C.byref(launches), C.byref(runs), C.byref(tiles)))
         return h, launches.value, runs.value, tiles.value
@@ -91,11 +91,11 @@ def oracle_world(orc, ts):
     # were written by an earlier launch
     (6_500, 9, {"roots": 3_000, "p": 0.99}),
 ])
-@pytest.mark.parametrize("use_plan", [False, True], ids=["levels", "plan"])
+@pytest.mark.parametrize("use_plan", [None, 1024, 512], ids=["levels", "plan1024", "plan512"])
 def test_propagate_matches_oracle(orc, n_nodes, seed, kw, use_plan):
     ts = scenes.tree_scene(n_nodes, seed, **kw)
     t = DeviceTree(ts)
-    plan = t.plan()[0] if use_plan else None
+    plan = t.plan(use_plan)[0] if use_plan else None
     t.propagate(plan=plan)
     if plan is not None:
         _lib.load().dpc_tree_plan_destroy(plan)
@@ -194,13 +194,14 @@ def test_dirty_propagation(orc):
     t.free()
 
 
-def test_plan_tiles_config2_tree_and_dirty_frames(orc):
+@pytest.mark.parametrize("tile_slots", [1024, 512])
+def test_plan_tiles_config2_tree_and_dirty_frames(orc, tile_slots):
     """The 1,000,000-node tree through the plan: its deep levels form a tile run;
     a full compute and then two partial ones (a few hundred dirty locals) give
     the oracle's world matrices and events bit for bit."""
     ts = scenes.tree_scene(1_000_000, 2)
     t = DeviceTree(ts)
-    plan, launches, runs, tiles = t.plan()
+    plan, launches, runs, tiles = t.plan(tile_slots)
     print(f"plan: {launches} launches, {runs} tile runs, {tiles} tiles for {len(ts.level_offsets) - 1} passes")
     assert runs >= 1 and launches < len(ts.level_offsets) - 1
     offsets, entries = ts.reference_levels()
@@ -237,3 +238,18 @@ def test_plan_tiles_config2_tree_and_dirty_frames(orc):
         t.dirty_world.fill_u32(0)
     lib.dpc_tree_plan_destroy(plan)
     t.free()
+
+
+def test_plan_rejects_bad_arguments():
+    import ctypes as C
+    lib = _lib.load()
+    h = C.c_void_p()
+    par = np.zeros(4, np.uint32)
+    off = np.array([1, 3, 5], np.uint64)  # ends beyond ld = 4
+    with pytest.raises(_lib.DpcError):
+        _lib.check(lib.dpc_tree_plan_create(par.ctypes.data, 4, off.ctypes.data, 2, 0, C.byref(h)))
+    off = np.array([1, 3, 4], np.uint64)
+    with pytest.raises(_lib.DpcError):  # an unsupported tile width
+        _lib.check(lib.dpc_tree_plan_create(par.ctypes.data, 4, off.ctypes.data, 2, 300, C.byref(h)))
+    _lib.check(lib.dpc_tree_plan_create(par.ctypes.data, 4, off.ctypes.data, 2, 0, C.byref(h)))
+    lib.dpc_tree_plan_destroy(h)
