@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const 
 // did not recompute keeps its value), so the passes are ordered by LDS-only
 // barriers.
 constexpr unsigned kTileSlots = 1024;  // the widest tile (one slot per thread)
-constexpr unsigned kDefaultTileSlots = 1024;
+constexpr unsigned kDefaultTileSlots = 512;  // two workgroups per CU; measured faster than 1024 on the 1M-node tree
 constexpr int kMaxTileLevels = 16;
 
 struct TileLevel {
@@ -303,7 +303,11 @@ __global__ __launch_bounds__(Slots) void tree_tile_kernel(const float4* __restri
 {
   __shared__ float4 rows[2][4][Slots];
   __shared__ uint8_t flags[2][Slots];
-  const TileLevel* t = tiles + static_cast<size_t>(blockIdx.x) * n_levels;
+  // the tile's ranges, read once: later reads come from LDS, so waiting for
+  // them never waits for the vector-memory loads in flight
+  __shared__ TileLevel t[kMaxTileLevels];
+  if (threadIdx.x < static_cast<unsigned>(n_levels)) t[threadIdx.x] = tiles[static_cast<size_t>(blockIdx.x) * n_levels + threadIdx.x];
+  __syncthreads();
   const int last = n_levels - 1;
   SlotIn A, B;
   load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[0], A);
