@@ -50,6 +50,7 @@ extern "C" {
 #define DPC_ERR_OVER_BUDGET 4
 #define DPC_ERR_NOT_FOUND 5
 #define DPC_ERR_STATE 6
+#define DPC_ERR_INTERNAL 7 /* a launch schedule failed its own consistency check (a library bug) */
 
 #define DPC_ABI_VERSION 1
 
@@ -208,13 +209,19 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
  * the level offsets as above) and keeps a device copy of the parent slots.
  * Consecutive passes in which every entry's parent lies in the previous pass
  * and which are ordered by parent slot become tile runs: one launch in which a
- * workgroup carries a slot range of the first pass and its descendants through
- * every pass of the run, with each pass's results in LDS for the next.  Small
- * passes share a single-workgroup launch; any other pass is one launch.
- * dpc_tree_propagate_plan has dpc_tree_propagate's semantics for that layout. */
+ * workgroup takes a slot range of the run's last pass together with every
+ * ancestor of it in the run's earlier passes, computes them pass by pass with
+ * each pass's results in LDS for the next, and writes the slots it owns
+ * (neighbouring tiles share at most one slot per pass and compute it both).
+ * Where no such run exists, small passes share a single-workgroup launch and
+ * any other pass is one launch.  tile_slots is the workgroup size of the tile
+ * runs (slots per step).  Every tile run is checked against the properties
+ * its kernel relies on before anything is uploaded (DPC_ERR_INTERNAL if one
+ * fails).  dpc_tree_propagate_plan has dpc_tree_propagate's semantics for
+ * that layout. */
 typedef struct dpc_tree_plan dpc_tree_plan;
 int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
-                         uint32_t tile_slots, dpc_tree_plan** plan);  /* tile_slots: 512, 1024 or 0 (default) */
+                         uint32_t tile_slots, dpc_tree_plan** plan);  /* tile_slots: 256, 512, 1024 or 0 (512) */
 int dpc_tree_plan_destroy(dpc_tree_plan* plan);
 /* launches per propagation, how many are tile runs, and their total tiles */
 int dpc_tree_plan_info(const dpc_tree_plan* plan, uint32_t* launches, uint32_t* tile_runs, uint64_t* tiles);

@@ -17,6 +17,7 @@ DPC_ERR_OUT_OF_MEMORY = 3
 DPC_ERR_OVER_BUDGET = 4
 DPC_ERR_NOT_FOUND = 5
 DPC_ERR_STATE = 6
+DPC_ERR_INTERNAL = 7
 
 
 class DpcError(RuntimeError):

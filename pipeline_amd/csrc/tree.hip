@@ -19,6 +19,7 @@
 
 #include "cull_math.hpp"
 #include "internal.hpp"
+#include "tree_plan.hpp"
 
 #pragma clang fp contract(off)
 
@@ -214,112 +215,170 @@ __global__ __launch_bounds__(kSmallThreads) void tree_small_levels_kernel(const 
 }
 
 // ---- tile runs -------------------------------------------------------------
-// A run of consecutive passes in which every entry's parent lies in the
-// previous pass and each pass is ordered by parent slot.  Then the children of
-// a contiguous slot range form a contiguous range of the next pass, and so on
-// down the run: a tile is a contiguous range of the run's first pass together
-// with those descendant ranges, at most kTileSlots slots per pass.  One
-// workgroup walks one tile pass by pass, keeping the previous pass's
-// recomputed rows and flags in LDS, where the next pass reads its parents.
-// Every parent read from memory is unchanged during the launch (a parent of
-// the run's first pass was written by an earlier launch; a parent the tile
-// did not recompute keeps its value), so the passes are ordered by LDS-only
-// barriers.
-constexpr unsigned kTileSlots = 1024;  // the widest tile (one slot per thread)
-constexpr unsigned kDefaultTileSlots = 512;  // two workgroups per CU; measured faster than 1024 on the 1M-node tree
-constexpr int kMaxTileLevels = 16;
+// One launch per run of linked passes (tree_plan.hpp describes the tiles and
+// their steps).  A workgroup takes one tile: step after step it loads one slot
+// per thread (the step's pieces may span several passes), then computes the
+// step's pieces in order; a kept pass's rows and flags go to the tile's LDS
+// ring, where the next pass reads its parents, and an LDS barrier closes each
+// kept pass.  The loads of the next step are issued before the current step is
+// computed, into the other of two register sets.  Slots the tile owns are
+// written to memory once per step, after its pieces.
+using tileplan::Seg;
+constexpr uint32_t kMaxSegs = 64;
+constexpr uint32_t kNoSeg = 0xFFFFFFFFu;
 
-struct TileLevel {
-  uint32_t first;  // first slot of the tile in this pass
-  uint32_t count;  // slots of the tile in this pass (<= kTileSlots)
+struct StepIn {
+  float4 l0, l1, l2, l3;
+  uint32_t p;
+  uint32_t dl;
+  uint32_t seg;   // this thread's piece (kNoSeg: idle in this step)
+  uint32_t slot;
 };
 
+// Finds this thread's piece among the pieces of the step that starts at piece
+// g and issues its loads (unconditional and clamped, as load_level).  Returns
+// the first piece of the next step (n when this is the last step).
 template <bool HasDirty>
-__device__ __forceinline__ void load_tile_slot(const float4* __restrict__ L, size_t ld,
-                                               const uint32_t* __restrict__ parent,
-                                               const uint8_t* __restrict__ dirty_local, TileLevel t, SlotIn& in)
+__device__ __forceinline__ uint32_t load_step(const float4* __restrict__ L, size_t ld,
+                                              const uint32_t* __restrict__ parent,
+                                              const uint8_t* __restrict__ dirty_local, const Seg* sg, uint32_t n,
+                                              uint32_t g, StepIn& in)
 {
-  // unconditional and clamped (slot 0 exists), as load_level
-  const uint64_t c = threadIdx.x < t.count ? static_cast<uint64_t>(t.first) + threadIdx.x : 0;
+  uint32_t mine = kNoSeg, slot = 0;
+  uint32_t j = g;
+  for (;;) {
+    const uint32_t pos = __builtin_amdgcn_readfirstlane(sg[j].pos);
+    const uint32_t count = __builtin_amdgcn_readfirstlane(sg[j].count);
+    const uint32_t first = __builtin_amdgcn_readfirstlane(sg[j].first);
+    const uint32_t flags = __builtin_amdgcn_readfirstlane(sg[j].flags);
+    const uint32_t d = threadIdx.x - pos;
+    if (d < count) {
+      mine = j;
+      slot = first + d;
+    }
+    ++j;
+    if ((flags & tileplan::kStepEnd) || j >= n) break;
+  }
+  in.seg = mine;
+  in.slot = slot;
+  const uint64_t c = mine != kNoSeg ? slot : 0;
   in.p = parent[c];
   in.dl = HasDirty ? dirty_local[c] : 1u;
   in.l0 = L[c];
   in.l1 = L[ld + c];
   in.l2 = L[2 * ld + c];
   in.l3 = L[3 * ld + c];
+  return j;
 }
 
-template <bool HasDirty, unsigned Slots>
-__device__ __forceinline__ void tile_level(const SlotIn& in, float4* W, size_t ld, uint8_t* dirty_world, TileLevel t,
-                                           TileLevel prev, bool first_pass, float4 (&lds_in)[4][Slots],
-                                           uint8_t (&flag_in)[Slots], float4 (&lds_out)[4][Slots],
-                                           uint8_t (&flag_out)[Slots])
+// Computes the pieces [a, b) of one step; each thread has at most one slot in
+// the step, written to the ring (kept passes) and, when the tile owns it, to
+// memory as soon as it is computed.
+template <unsigned MaxRing>
+__device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32_t a, uint32_t b, uint32_t ring_len,
+                                         float4* W, size_t ld, uint8_t* dirty_world, float4 (&rows)[4][MaxRing],
+                                         uint8_t (&flag)[MaxRing])
 {
-  if (threadIdx.x >= t.count) return;
-  const uint64_t s = static_cast<uint64_t>(t.first) + threadIdx.x;
-  const uint64_t p = in.p;
-  const bool valid = p < ld;  // a parent outside the arrays is never read (as in propagate_slot)
-  // a parent of a later pass lies in the tile's previous range (by construction)
-  const bool staged = valid && !first_pass && p >= prev.first && p < static_cast<uint64_t>(prev.first) + prev.count;
-  const uint32_t i = staged ? static_cast<uint32_t>(p - prev.first) : 0u;
-  const uint32_t dp_lds = staged ? flag_in[i] : 0u;
-  bool dirty = false;
-  float4 p0, p1, p2, p3;
-  if (dp_lds) {
-    p0 = lds_in[0][i];
-    p1 = lds_in[1][i];
-    p2 = lds_in[2][i];
-    p3 = lds_in[3][i];
-    dirty = true;
-  } else if (valid && (!staged || in.dl)) {
-    uint32_t dp = 0;
-    load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
-    dirty = (in.dl | (staged ? 0u : dp)) != 0;
+  for (uint32_t j = a; j < b; ++j) {
+    const uint32_t flags = __builtin_amdgcn_readfirstlane(sg[j].flags);
+    if (in.seg == j) {
+      const uint32_t pfirst = __builtin_amdgcn_readfirstlane(sg[j].pfirst);
+      const uint32_t pcount = __builtin_amdgcn_readfirstlane(sg[j].pcount);
+      const uint32_t pring = __builtin_amdgcn_readfirstlane(sg[j].pring);
+      const uint64_t p = in.p;
+      const bool valid = p < ld;  // a parent outside the arrays is never read (as in propagate_slot)
+      const uint32_t d = static_cast<uint32_t>(p) - pfirst;
+      const bool staged = valid && !(flags & tileplan::kFromMemory) && d < pcount;
+      uint32_t i = pring + d;
+      i = i >= ring_len ? i - ring_len : i;
+      i = staged && i < MaxRing ? i : 0u;
+      const uint32_t dp_lds = staged ? flag[i] : 0u;
+      bool dirty = false;
+      float4 p0, p1, p2, p3;
+      if (dp_lds) {  // the parent was recomputed by this tile: its rows are in the ring
+        p0 = rows[0][i];
+        p1 = rows[1][i];
+        p2 = rows[2][i];
+        p3 = rows[3][i];
+        dirty = true;
+      } else if (valid && (!staged || in.dl)) {
+        // a kept parent that the tile left unchanged has dirty_world == 0
+        uint32_t dp = 0;
+        load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
+        dirty = (in.dl | (staged ? 0u : dp)) != 0;
+      }
+      const bool kept = (flags & tileplan::kStaged) != 0;
+      uint32_t o = 0;
+      if (kept) {
+        const uint32_t ring = __builtin_amdgcn_readfirstlane(sg[j].ring);
+        const uint32_t pos = __builtin_amdgcn_readfirstlane(sg[j].pos);
+        o = ring + (threadIdx.x - pos);
+        o = o >= ring_len ? o - ring_len : o;
+        o = o < MaxRing ? o : 0u;  // the plan keeps positions below its ring length (at most MaxRing)
+        flag[o] = dirty ? 1 : 0;
+      }
+      if (dirty) {
+        const bool own = in.slot < __builtin_amdgcn_readfirstlane(sg[j].own_end);
+        const uint64_t s = in.slot;
+        const float4 w0 = rowmat(in.l0, p0, p1, p2, p3);
+        const float4 w1 = rowmat(in.l1, p0, p1, p2, p3);
+        const float4 w2 = rowmat(in.l2, p0, p1, p2, p3);
+        const float4 w3 = rowmat(in.l3, p0, p1, p2, p3);
+        if (kept) {
+          rows[0][o] = w0;
+          rows[1][o] = w1;
+          rows[2][o] = w2;
+          rows[3][o] = w3;
+        }
+        if (own) {
+          W[s] = w0;
+          W[ld + s] = w1;
+          W[2 * ld + s] = w2;
+          W[3 * ld + s] = w3;
+          dirty_world[s] = 1;
+        }
+      }
+    }
+    if (flags & tileplan::kBarrier) lds_barrier();
   }
-  if (dirty) {
-    const float4 w0 = rowmat(in.l0, p0, p1, p2, p3);
-    const float4 w1 = rowmat(in.l1, p0, p1, p2, p3);
-    const float4 w2 = rowmat(in.l2, p0, p1, p2, p3);
-    const float4 w3 = rowmat(in.l3, p0, p1, p2, p3);
-    W[s] = w0;
-    W[ld + s] = w1;
-    W[2 * ld + s] = w2;
-    W[3 * ld + s] = w3;
-    dirty_world[s] = 1;
-    lds_out[0][threadIdx.x] = w0;
-    lds_out[1][threadIdx.x] = w1;
-    lds_out[2][threadIdx.x] = w2;
-    lds_out[3][threadIdx.x] = w3;
-  }
-  flag_out[threadIdx.x] = dirty ? 1 : 0;
 }
 
-template <bool HasDirty, unsigned Slots>
-__global__ __launch_bounds__(Slots) void tree_tile_kernel(const float4* __restrict__ L, float4* W, size_t ld,
-                                                          const uint32_t* __restrict__ parent,
-                                                          const TileLevel* __restrict__ tiles, int n_levels,
-                                                          const uint8_t* __restrict__ dirty_local,
-                                                          uint8_t* dirty_world)
+// LDS ring positions per tile: four workgroups of 256 threads per CU (39 KB
+// each), two of 512 (77 KB) or one of 1024 (133 KB).  The plan may use fewer.
+constexpr unsigned tile_max_ring(unsigned threads)
 {
-  __shared__ float4 rows[2][4][Slots];
-  __shared__ uint8_t flags[2][Slots];
-  // the tile's ranges, read once: later reads come from LDS, so waiting for
-  // them never waits for the vector-memory loads in flight
-  __shared__ TileLevel t[kMaxTileLevels];
-  if (threadIdx.x < static_cast<unsigned>(n_levels)) t[threadIdx.x] = tiles[static_cast<size_t>(blockIdx.x) * n_levels + threadIdx.x];
+  return threads == 256 ? 576 : threads == 512 ? 1152 : 2048;
+}
+
+template <bool HasDirty, unsigned Threads>
+__global__ __launch_bounds__(Threads) void tree_tile_kernel(const float4* __restrict__ L, float4* W, size_t ld,
+                                                            const uint32_t* __restrict__ parent,
+                                                            const Seg* __restrict__ segs,
+                                                            const uint32_t* __restrict__ tile_seg, uint32_t ring_len,
+                                                            const uint8_t* __restrict__ dirty_local,
+                                                            uint8_t* dirty_world)
+{
+  constexpr unsigned MaxRing = tile_max_ring(Threads);
+  __shared__ float4 rows[4][MaxRing];
+  __shared__ uint8_t flag[MaxRing];
+  __shared__ Seg sg[kMaxSegs];
+  const uint32_t g0 = tile_seg[blockIdx.x];
+  const uint32_t n = min(tile_seg[blockIdx.x + 1] - g0, kMaxSegs);
+  for (uint32_t i = threadIdx.x; i < n; i += Threads) sg[i] = segs[g0 + i];
   __syncthreads();
-  const int last = n_levels - 1;
-  SlotIn A, B;
-  load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[0], A);
-  for (int l = 0; l < n_levels; l += 2) {
-    load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[l + 1 < n_levels ? l + 1 : last], B);
-    tile_level<HasDirty, Slots>(A, W, ld, dirty_world, t[l], t[l > 0 ? l - 1 : 0], l == 0, rows[1], flags[1],
-                                rows[0], flags[0]);
-    lds_barrier();
-    if (l + 1 >= n_levels) break;
-    load_tile_slot<HasDirty>(L, ld, parent, dirty_local, t[l + 2 < n_levels ? l + 2 : last], A);
-    tile_level<HasDirty, Slots>(B, W, ld, dirty_world, t[l + 1], t[l], false, rows[0], flags[0], rows[1], flags[1]);
-    lds_barrier();
+  StepIn A, B;
+  uint32_t a = 0;
+  uint32_t b = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, a, A);
+  for (;;) {
+    // the next step's inputs (past the last step: this step again, unused)
+    const uint32_t c = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, b < n ? b : a, B);
+    run_step<MaxRing>(A, sg, a, b, ring_len, W, ld, dirty_world, rows, flag);
+    if (b >= n) break;
+    const uint32_t d = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, c < n ? c : b, A);
+    run_step<MaxRing>(B, sg, b, c, ring_len, W, ld, dirty_world, rows, flag);
+    if (c >= n) break;
+    a = c;
+    b = d;
   }
 }
 
@@ -400,113 +459,55 @@ using namespace dpc;
 
 // The launch schedule of one tree layout (dpc_tree_plan_create).
 struct dpc_tree_plan {
-  enum Kind { kSmall, kLevel, kTiles };
   struct Step {
-    Kind kind;
-    LevelRun run;          // kSmall
-    uint64_t s0 = 0, s1 = 0;  // kLevel
-    int n_levels = 0;      // kTiles
-    uint32_t n_tiles = 0;
-    TileLevel* d_tiles = nullptr;
+    tileplan::Step::Kind kind;
+    LevelRun run{};            // kSmall
+    uint64_t s0 = 0, s1 = 0;   // kLevel
+    uint32_t n_tiles = 0;      // kTiles
+    uint32_t ring = 0;
+    Seg* d_segs = nullptr;
+    uint32_t* d_tile_seg = nullptr;
   };
   size_t ld = 0;
-  uint32_t tile_slots = kTileSlots;
+  uint32_t threads = 0;  // of a tile workgroup
   uint32_t* d_parent = nullptr;
   std::vector<Step> steps;
   uint64_t tiles = 0;
   ~dpc_tree_plan()
   {
-    for (auto& st : steps)
-      if (st.d_tiles) (void)hipFree(st.d_tiles);
+    for (auto& st : steps) {
+      if (st.d_segs) (void)hipFree(st.d_segs);
+      if (st.d_tile_seg) (void)hipFree(st.d_tile_seg);
+    }
     if (d_parent) (void)hipFree(d_parent);
   }
 };
 
 namespace {
 
-// Host side of the plan: which consecutive passes form tile runs, and their tiles.
-struct PlanBuilder {
-  const uint32_t* par;
-  const uint64_t* off;
-  size_t n;  // passes
-  uint32_t width = kTileSlots;  // slots of a tile per pass
-  std::vector<uint8_t> link;  // link[j]: every parent of pass j is in pass j-1 and pass j is ordered by parent
+constexpr uint32_t kDefaultTileThreads = 512;
+constexpr size_t kMaxRunPasses = 32;
 
-  PlanBuilder(const uint32_t* parent, const uint64_t* offsets, size_t n_levels) : par(parent), off(offsets), n(n_levels)
-  {
-    link.assign(n, 0);
-    for (size_t j = 1; j < n; ++j) {
-      bool ok = off[j + 1] > off[j];
-      for (uint64_t s = off[j]; ok && s < off[j + 1]; ++s) {
-        ok = par[s] >= off[j - 1] && par[s] < off[j];
-        if (ok && s > off[j]) ok = par[s] >= par[s - 1];
-      }
-      link[j] = ok;
-    }
+template <bool HasDirty>
+void launch_tiles(const dpc_tree_plan& plan, const dpc_tree_plan::Step& st, const float4* L, float4* W,
+                  const uint8_t* dirty_local, uint8_t* dirty_world, hipStream_t s)
+{
+  const dim3 grid(st.n_tiles);
+  switch (plan.threads) {
+    case 256:
+      hipLaunchKernelGGL((tree_tile_kernel<HasDirty, 256>), grid, dim3(256), 0, s, L, W, plan.ld, plan.d_parent,
+                         st.d_segs, st.d_tile_seg, st.ring, dirty_local, dirty_world);
+      break;
+    case 1024:
+      hipLaunchKernelGGL((tree_tile_kernel<HasDirty, 1024>), grid, dim3(1024), 0, s, L, W, plan.ld, plan.d_parent,
+                         st.d_segs, st.d_tile_seg, st.ring, dirty_local, dirty_world);
+      break;
+    default:
+      hipLaunchKernelGGL((tree_tile_kernel<HasDirty, 512>), grid, dim3(512), 0, s, L, W, plan.ld, plan.d_parent,
+                         st.d_segs, st.d_tile_seg, st.ring, dirty_local, dirty_world);
+      break;
   }
-
-  // first[v - off[j-1]] for v in [off[j-1], off[j]]: the first slot of pass j
-  // whose parent is >= v (pass j ordered by parent).
-  std::vector<uint32_t> first_child(size_t j) const
-  {
-    const uint64_t a = off[j - 1], b = off[j];
-    std::vector<uint32_t> f(b - a + 1);
-    uint64_t c = off[j];
-    for (uint64_t v = a; v <= b; ++v) {
-      while (c < off[j + 1] && par[c] < v) ++c;
-      f[v - a] = static_cast<uint32_t>(c);
-    }
-    return f;
-  }
-
-  // Descendant boundaries of the nodes of pass l: cb[k][x - off[l]] is the first
-  // slot of pass l + k below node x (x = off[l + 1] gives the end of the pass).
-  // Extends while every link holds and no single node has more than `width`
-  // descendants in a pass; returns the number of passes covered.
-  size_t descend(size_t l, std::vector<std::vector<uint32_t>>& cb) const
-  {
-    cb.clear();
-    std::vector<uint32_t> cur(off[l + 1] - off[l] + 1);
-    for (size_t x = 0; x < cur.size(); ++x) cur[x] = static_cast<uint32_t>(off[l] + x);
-    cb.push_back(cur);
-    for (size_t j = l + 1; j < n && cb.size() < static_cast<size_t>(kMaxTileLevels); ++j) {
-      if (!link[j]) break;
-      const std::vector<uint32_t> f = first_child(j);
-      std::vector<uint32_t> next(cur.size());
-      uint32_t widest = 0;
-      for (size_t x = 0; x < cur.size(); ++x) {
-        next[x] = f[cur[x] - off[j - 1]];
-        if (x) widest = std::max(widest, next[x] - next[x - 1]);
-      }
-      if (widest > width) break;
-      cb.push_back(next);
-      cur.swap(next);
-    }
-    return cb.size();
-  }
-
-  // Tiles of the run of passes [l, l + cb.size()): greedy contiguous ranges of
-  // pass l, each at most `width` slots in every pass.
-  std::vector<TileLevel> tiles(size_t l, const std::vector<std::vector<uint32_t>>& cb) const
-  {
-    const size_t levels = cb.size();
-    const size_t nodes = off[l + 1] - off[l];
-    std::vector<TileLevel> out;
-    size_t x0 = 0;
-    while (x0 < nodes) {
-      size_t x1 = x0 + 1;
-      while (x1 < nodes) {
-        bool fits = true;
-        for (size_t k = 0; k < levels && fits; ++k) fits = cb[k][x1 + 1] - cb[k][x0] <= width;
-        if (!fits) break;
-        ++x1;
-      }
-      for (size_t k = 0; k < levels; ++k) out.push_back(TileLevel{cb[k][x0], cb[k][x1] - cb[k][x0]});
-      x0 = x1;
-    }
-    return out;
-  }
-};
+}
 
 }  // namespace
 
@@ -601,66 +602,72 @@ int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_
       return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: level offsets decrease at level %zu", l);
   if (h_level_offsets[n_levels] > ld)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: levels end beyond ld");
-  if (tile_slots != 0 && tile_slots != 512 && tile_slots != 1024)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: tile_slots must be 0, 512 or 1024");
+  if (tile_slots != 0 && tile_slots != 256 && tile_slots != 512 && tile_slots != 1024)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: tile_slots must be 0, 256, 512 or 1024");
   auto plan = std::make_unique<dpc_tree_plan>();
   plan->ld = ld;
-  plan->tile_slots = tile_slots ? tile_slots : kDefaultTileSlots;
+  plan->threads = tile_slots ? tile_slots : kDefaultTileThreads;
+  const uint32_t max_ring = tile_max_ring(plan->threads);
+  // workgroups the device holds at once: CUs x tile workgroups per CU
+  int dev = 0, cus = 0, per_cu = 0;
+  DPC_HIP_TRY(hipGetDevice(&dev), "dpc_tree_plan_create: device");
+  DPC_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "dpc_tree_plan_create: device");
+  switch (plan->threads) {
+    case 256:
+      DPC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tree_tile_kernel<true, 256>, 256, 0),
+                  "dpc_tree_plan_create: occupancy");
+      break;
+    case 1024:
+      DPC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tree_tile_kernel<true, 1024>, 1024, 0),
+                  "dpc_tree_plan_create: occupancy");
+      break;
+    default:
+      DPC_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tree_tile_kernel<true, 512>, 512, 0),
+                  "dpc_tree_plan_create: occupancy");
+      break;
+  }
+  const uint64_t capacity = static_cast<uint64_t>(std::max(cus, 1)) * static_cast<uint64_t>(std::max(per_cu, 1));
   DPC_HIP_TRY(hipMalloc(&plan->d_parent, ld * sizeof(uint32_t)), "dpc_tree_plan_create: parents");
   DPC_HIP_TRY(hipMemcpy(plan->d_parent, h_parent_slot, ld * sizeof(uint32_t), hipMemcpyHostToDevice),
               "dpc_tree_plan_create: parents");
-  PlanBuilder pb(h_parent_slot, h_level_offsets, n_levels);
-  pb.width = plan->tile_slots;
-  const uint64_t* off = h_level_offsets;
-  std::vector<std::vector<uint32_t>> cb, cb_next;
-  size_t l = 0;
-  while (l < n_levels) {
-    const uint64_t size = off[l + 1] - off[l];
-    if (size == 0) {
-      ++l;
-      continue;
-    }
-    if (size <= kLdsNodes) {  // consecutive small passes: one single-workgroup run
-      dpc_tree_plan::Step st{};
-      st.kind = dpc_tree_plan::kSmall;
-      st.run.n = 0;
-      st.run.off[0] = off[l];
-      while (l < n_levels && st.run.n < kMaxLevelsPerSmall && off[l + 1] - st.run.off[0] <= kLdsNodes) {
-        st.run.off[st.run.n + 1] = off[l + 1];
-        ++st.run.n;
-        ++l;
-      }
-      plan->steps.push_back(st);
-      continue;
-    }
-    // a tile run, unless starting at the next pass reaches deeper
-    const size_t reach = pb.descend(l, cb);
-    bool tile = reach >= 2;
-    if (tile && l + reach < n_levels && l + 1 < n_levels && off[l + 2] - off[l + 1] > kLdsNodes) {
-      const size_t reach_next = pb.descend(l + 1, cb_next);
-      if (l + 1 + reach_next > l + reach) tile = false;
-    }
-    if (tile) {
-      pb.descend(l, cb);  // cb may have been reused above
-      const std::vector<TileLevel> t = pb.tiles(l, cb);
-      dpc_tree_plan::Step st{};
-      st.kind = dpc_tree_plan::kTiles;
-      st.n_levels = static_cast<int>(cb.size());
-      st.n_tiles = static_cast<uint32_t>(t.size() / cb.size());
-      DPC_HIP_TRY(hipMalloc(&st.d_tiles, t.size() * sizeof(TileLevel)), "dpc_tree_plan_create: tiles");
-      plan->steps.push_back(st);
-      DPC_HIP_TRY(hipMemcpy(st.d_tiles, t.data(), t.size() * sizeof(TileLevel), hipMemcpyHostToDevice),
+  // a parent slot at or beyond ld is never read; the planner follows parents of
+  // linked passes only, which lie inside the levels
+  const std::vector<tileplan::Step> sched =
+      tileplan::schedule(h_parent_slot, h_level_offsets, n_levels, plan->threads, max_ring, capacity, kLdsNodes,
+                         kMaxLevelsPerSmall, kMaxRunPasses);
+  for (const auto& ts : sched) {
+    dpc_tree_plan::Step st{};
+    st.kind = ts.kind;
+    if (ts.kind == tileplan::Step::kSmall) {
+      st.run.n = static_cast<int>(ts.r - ts.l);
+      for (size_t k = ts.l; k <= ts.r; ++k) st.run.off[k - ts.l] = h_level_offsets[k];
+    } else if (ts.kind == tileplan::Step::kLevel) {
+      st.s0 = h_level_offsets[ts.l];
+      st.s1 = h_level_offsets[ts.r];
+    } else {
+      const std::string why =
+          ts.run.ring == 0 || ts.run.ring > max_ring
+              ? std::string("ring larger than the kernel's LDS")
+              : tileplan::check_run(h_parent_slot, h_level_offsets, tileplan::limits_for(plan->threads, ts.run.ring),
+                                    ts.run);
+      if (!why.empty())
+        return set_error(DPC_ERR_INTERNAL, "dpc_tree_plan_create: tile run of passes %zu-%zu: %s", ts.l, ts.r - 1,
+                         why.c_str());
+      st.n_tiles = static_cast<uint32_t>(ts.run.tiles());
+      st.ring = ts.run.ring;
+      plan->steps.push_back(st);  // owns the buffers from here on
+      auto& ps = plan->steps.back();
+      DPC_HIP_TRY(hipMalloc(&ps.d_segs, ts.run.segs.size() * sizeof(Seg)), "dpc_tree_plan_create: tiles");
+      DPC_HIP_TRY(hipMalloc(&ps.d_tile_seg, ts.run.tile_seg.size() * sizeof(uint32_t)), "dpc_tree_plan_create: tiles");
+      DPC_HIP_TRY(hipMemcpy(ps.d_segs, ts.run.segs.data(), ts.run.segs.size() * sizeof(Seg), hipMemcpyHostToDevice),
+                  "dpc_tree_plan_create: tiles");
+      DPC_HIP_TRY(hipMemcpy(ps.d_tile_seg, ts.run.tile_seg.data(), ts.run.tile_seg.size() * sizeof(uint32_t),
+                            hipMemcpyHostToDevice),
                   "dpc_tree_plan_create: tiles");
       plan->tiles += st.n_tiles;
-      l += cb.size();
       continue;
     }
-    dpc_tree_plan::Step st{};
-    st.kind = dpc_tree_plan::kLevel;
-    st.s0 = off[l];
-    st.s1 = off[l + 1];
     plan->steps.push_back(st);
-    ++l;
   }
   *out = plan.release();
   return DPC_OK;
@@ -676,7 +683,7 @@ int dpc_tree_plan_info(const dpc_tree_plan* plan, uint32_t* launches, uint32_t* 
 {
   if (!plan) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_info: null plan");
   uint32_t runs = 0;
-  for (const auto& st : plan->steps) runs += st.kind == dpc_tree_plan::kTiles ? 1 : 0;
+  for (const auto& st : plan->steps) runs += st.kind == tileplan::Step::kTiles ? 1 : 0;
   if (launches) *launches = static_cast<uint32_t>(plan->steps.size());
   if (tile_runs) *tile_runs = runs;
   if (tiles) *tiles = plan->tiles;
@@ -696,7 +703,7 @@ int dpc_tree_propagate_plan(const dpc_tree_plan* plan, const float* d_local, flo
   hipStream_t s = as_stream(stream);
   for (const auto& st : plan->steps) {
     switch (st.kind) {
-      case dpc_tree_plan::kSmall:
+      case tileplan::Step::kSmall:
         if (d_dirty_local)
           hipLaunchKernelGGL(tree_small_levels_kernel<true>, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld,
                              plan->d_parent, st.run, d_dirty_local, d_dirty_world);
@@ -704,27 +711,16 @@ int dpc_tree_propagate_plan(const dpc_tree_plan* plan, const float* d_local, flo
           hipLaunchKernelGGL(tree_small_levels_kernel<false>, dim3(1), dim3(kSmallThreads), 0, s, L, W, ld,
                              plan->d_parent, st.run, d_dirty_local, d_dirty_world);
         break;
-      case dpc_tree_plan::kLevel:
+      case tileplan::Step::kLevel:
         hipLaunchKernelGGL(tree_level_kernel, dim3(static_cast<unsigned>((st.s1 - st.s0 + kTreeThreads - 1) / kTreeThreads)),
                            dim3(kTreeThreads), 0, s, L, W, ld, plan->d_parent, st.s0, st.s1, d_dirty_local,
                            d_dirty_world);
         break;
-      case dpc_tree_plan::kTiles:
-        if (plan->tile_slots == 512) {
-          if (d_dirty_local)
-            hipLaunchKernelGGL((tree_tile_kernel<true, 512>), dim3(st.n_tiles), dim3(512), 0, s, L, W, ld,
-                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
-          else
-            hipLaunchKernelGGL((tree_tile_kernel<false, 512>), dim3(st.n_tiles), dim3(512), 0, s, L, W, ld,
-                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
-        } else {
-          if (d_dirty_local)
-            hipLaunchKernelGGL((tree_tile_kernel<true, 1024>), dim3(st.n_tiles), dim3(1024), 0, s, L, W, ld,
-                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
-          else
-            hipLaunchKernelGGL((tree_tile_kernel<false, 1024>), dim3(st.n_tiles), dim3(1024), 0, s, L, W, ld,
-                               plan->d_parent, st.d_tiles, st.n_levels, d_dirty_local, d_dirty_world);
-        }
+      case tileplan::Step::kTiles:
+        if (d_dirty_local)
+          launch_tiles<true>(*plan, st, L, W, d_dirty_local, d_dirty_world, s);
+        else
+          launch_tiles<false>(*plan, st, L, W, d_dirty_local, d_dirty_world, s);
         break;
     }
     if (int rc = check_launch("dpc_tree_propagate_plan")) return rc;

@@ -1,6 +1,7 @@
 """Runs the C++ tests of the host mirrors (tests/cpp): the transform tree's
-bookkeeping against the reference's Tree trace (CPU), the GPU tree, and the
-HIP culling manager against the CPU manager restatement (GPU)."""
+bookkeeping against the reference's Tree trace (CPU), the propagation plan's
+tile runs against an emulation (CPU), the GPU tree, and the HIP culling
+manager against the CPU manager restatement (GPU)."""
 import subprocess
 
 import pytest
@@ -23,6 +24,14 @@ def run(name, *args, timeout=300):
 def test_tree_layout_matches_reference_trace():
     out = run("tree_layout_test", str(GOLDEN / "ref_tree_trace.bin"))
     assert "0 failures" in out
+
+
+def test_tree_plan_tiles_emulated():
+    """The propagation plan's tile runs pass their consistency check, and an
+    emulation of the tile kernel from their descriptors equals pass-by-pass
+    propagation (random, index-reuse and adversarial trees, the 1M-node tree)."""
+    out = run("tree_plan_test", timeout=600)
+    assert "tree plan: 0 failures" in out
 
 
 @pytest.mark.gpu

@@ -91,7 +91,7 @@ def oracle_world(orc, ts):
     # were written by an earlier launch
     (6_500, 9, {"roots": 3_000, "p": 0.99}),
 ])
-@pytest.mark.parametrize("use_plan", [None, 1024, 512], ids=["levels", "plan1024", "plan512"])
+@pytest.mark.parametrize("use_plan", [None, 1024, 512, 256], ids=["levels", "plan1024", "plan512", "plan256"])
 def test_propagate_matches_oracle(orc, n_nodes, seed, kw, use_plan):
     ts = scenes.tree_scene(n_nodes, seed, **kw)
     t = DeviceTree(ts)
@@ -194,7 +194,7 @@ def test_dirty_propagation(orc):
     t.free()
 
 
-@pytest.mark.parametrize("tile_slots", [1024, 512])
+@pytest.mark.parametrize("tile_slots", [1024, 512, 256])
 def test_plan_tiles_config2_tree_and_dirty_frames(orc, tile_slots):
     """The 1,000,000-node tree through the plan: its deep levels form a tile run;
     a full compute and then two partial ones (a few hundred dirty locals) give
