@@ -233,34 +233,69 @@ struct StepIn {
   uint32_t dl;
   uint32_t seg;   // this thread's piece (kNoSeg: idle in this step)
   uint32_t slot;
+  uint32_t row;   // the matrix row this thread computes in a kQuad piece (0 otherwise)
 };
 
+// A piece's fields as wave-uniform values (scalar registers): the first 16
+// bytes of a Seg in one LDS load, and the parent range in another.
+struct SegHead {
+  uint32_t first, count, pos, ring, flags, own_end;
+};
+
+__device__ __forceinline__ SegHead decode_head(const uint4 h)
+{
+  SegHead r;
+  r.first = __builtin_amdgcn_readfirstlane(h.x);
+  const uint32_t cp = __builtin_amdgcn_readfirstlane(h.y);  // count | pos << 16
+  const uint32_t rf = __builtin_amdgcn_readfirstlane(h.z);  // ring | flags << 16
+  r.count = cp & 0xFFFFu;
+  r.pos = cp >> 16;
+  r.ring = rf & 0xFFFFu;
+  r.flags = rf >> 16;
+  r.own_end = __builtin_amdgcn_readfirstlane(h.w);
+  return r;
+}
+
+__device__ __forceinline__ SegHead seg_head(const Seg* sg, uint32_t j)
+{
+  return decode_head(*reinterpret_cast<const uint4*>(&sg[j]));
+}
+
 // Finds this thread's piece among the pieces of the step that starts at piece
-// g and issues its loads (unconditional and clamped, as load_level).  Returns
-// the first piece of the next step (n when this is the last step).
+// g (four pieces per round of LDS loads) and issues its loads (unconditional
+// and clamped, as load_level).  Returns the first piece of the next step (n
+// when this is the last step).
 template <bool HasDirty>
 __device__ __forceinline__ uint32_t load_step(const float4* __restrict__ L, size_t ld,
                                               const uint32_t* __restrict__ parent,
                                               const uint8_t* __restrict__ dirty_local, const Seg* sg, uint32_t n,
                                               uint32_t g, StepIn& in)
 {
-  uint32_t mine = kNoSeg, slot = 0;
-  uint32_t j = g;
-  for (;;) {
-    const uint32_t pos = __builtin_amdgcn_readfirstlane(sg[j].pos);
-    const uint32_t count = __builtin_amdgcn_readfirstlane(sg[j].count);
-    const uint32_t first = __builtin_amdgcn_readfirstlane(sg[j].first);
-    const uint32_t flags = __builtin_amdgcn_readfirstlane(sg[j].flags);
-    const uint32_t d = threadIdx.x - pos;
-    if (d < count) {
-      mine = j;
-      slot = first + d;
+  uint32_t mine = kNoSeg, slot = 0, row = 0;
+  uint32_t next = g;
+  bool ended = false;
+  for (uint32_t j = g; !ended; j += 4) {
+    SegHead h[4];
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) h[q] = seg_head(sg, min(j + q, n - 1));
+#pragma unroll
+    for (uint32_t q = 0; q < 4; ++q) {
+      if (ended) continue;
+      const uint32_t d = threadIdx.x - h[q].pos;
+      const uint32_t sh = (h[q].flags & tileplan::kQuad) ? 2u : 0u;  // threads per slot: 1 << sh
+      if (d < (h[q].count << sh)) {
+        mine = j + q;
+        slot = h[q].first + (d >> sh);
+        row = d & ((1u << sh) - 1u);
+      }
+      next = j + q + 1;
+      ended = (h[q].flags & tileplan::kStepEnd) || next >= n;
     }
-    ++j;
-    if ((flags & tileplan::kStepEnd) || j >= n) break;
   }
+  next = min(next, n);
   in.seg = mine;
   in.slot = slot;
+  in.row = row;
   const uint64_t c = mine != kNoSeg ? slot : 0;
   in.p = parent[c];
   in.dl = HasDirty ? dirty_local[c] : 1u;
@@ -268,38 +303,79 @@ __device__ __forceinline__ uint32_t load_step(const float4* __restrict__ L, size
   in.l1 = L[ld + c];
   in.l2 = L[2 * ld + c];
   in.l3 = L[3 * ld + c];
-  return j;
+  return next;
 }
 
-// Computes the pieces [a, b) of one step; each thread has at most one slot in
-// the step, written to the ring (kept passes) and, when the tile owns it, to
-// memory as soon as it is computed.
+// Buffer stores with an out-of-range offset are dropped by the hardware: a
+// lane that must not write gets kNoStore, so every step issues the same store
+// instructions on every path and the compiler can count the loads still in
+// flight exactly (a store behind a branch makes it wait for all of them).
+constexpr uint32_t kNoStore = 0x80000000u;
+constexpr uint64_t kMaxBufferSlots = 1ull << 27;  // row planes addressed with 32-bit byte offsets below kNoStore
+constexpr int kRawBufferDword3 = 0x00020000;      // gfx9 raw buffer: 32-bit data format, no swizzle
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void store_row(float4 v, __amdgpu_buffer_rsrc_t r, uint32_t offset)
+{
+  u32x4 u;
+  u.x = __float_as_uint(v.x);
+  u.y = __float_as_uint(v.y);
+  u.z = __float_as_uint(v.z);
+  u.w = __float_as_uint(v.w);
+  __builtin_amdgcn_raw_buffer_store_b128(u, r, offset, 0, 0);
+}
+
+// Row in.row of the local matrix, selected with bit masks (exact for every
+// bit pattern, and kept in registers: a select the compiler turns into an
+// indexed access would put the four rows in scratch memory).
+__device__ __forceinline__ float4 pick_row(const StepIn& in)
+{
+  const uint32_t m0 = in.row == 0 ? ~0u : 0u, m1 = in.row == 1 ? ~0u : 0u;
+  const uint32_t m2 = in.row == 2 ? ~0u : 0u, m3 = in.row == 3 ? ~0u : 0u;
+  auto pick = [&](float a, float b, float c, float d) {
+    return __uint_as_float((__float_as_uint(a) & m0) | (__float_as_uint(b) & m1) | (__float_as_uint(c) & m2) |
+                           (__float_as_uint(d) & m3));
+  };
+  return make_float4(pick(in.l0.x, in.l1.x, in.l2.x, in.l3.x), pick(in.l0.y, in.l1.y, in.l2.y, in.l3.y),
+                     pick(in.l0.z, in.l1.z, in.l2.z, in.l3.z), pick(in.l0.w, in.l1.w, in.l2.w, in.l3.w));
+}
+
+// Computes the pieces [a, b) of one step.  Each thread has at most one slot in
+// the step: its rows go to the ring (kept passes) inside the loop, and to
+// memory (when the tile owns the slot and it changed) after the loop.
 template <unsigned MaxRing>
 __device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32_t a, uint32_t b, uint32_t ring_len,
-                                         float4* W, size_t ld, uint8_t* dirty_world, float4 (&rows)[4][MaxRing],
-                                         uint8_t (&flag)[MaxRing])
+                                         const float4* W, size_t ld, const uint8_t* dirty_world,
+                                         const __amdgpu_buffer_rsrc_t (&rw)[4], __amdgpu_buffer_rsrc_t rdw,
+                                         float4 (&rows)[4][MaxRing], uint8_t (&flag)[MaxRing])
 {
+  bool store = false, quad = false;
+  float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0, w2 = w0, w3 = w0;
+  // a < b <= n for every step of a valid plan; b <= n always (load_step)
+  SegHead h = seg_head(sg, a < b ? a : 0u);
   for (uint32_t j = a; j < b; ++j) {
-    const uint32_t flags = __builtin_amdgcn_readfirstlane(sg[j].flags);
+    // the next piece's descriptor, read while this one is computed
+    const uint4 hn = *reinterpret_cast<const uint4*>(&sg[j + 1 < b ? j + 1 : j]);
     if (in.seg == j) {
-      const uint32_t pfirst = __builtin_amdgcn_readfirstlane(sg[j].pfirst);
-      const uint32_t pcount = __builtin_amdgcn_readfirstlane(sg[j].pcount);
-      const uint32_t pring = __builtin_amdgcn_readfirstlane(sg[j].pring);
+      const uint2 pr = *reinterpret_cast<const uint2*>(&sg[j].pfirst);
+      const uint32_t pfirst = __builtin_amdgcn_readfirstlane(pr.x);
+      const uint32_t pp = __builtin_amdgcn_readfirstlane(pr.y);  // pcount | pring << 16
+      const uint32_t pcount = pp & 0xFFFFu, pring = pp >> 16;
       const uint64_t p = in.p;
       const bool valid = p < ld;  // a parent outside the arrays is never read (as in propagate_slot)
       const uint32_t d = static_cast<uint32_t>(p) - pfirst;
-      const bool staged = valid && !(flags & tileplan::kFromMemory) && d < pcount;
+      const bool staged = valid && !(h.flags & tileplan::kFromMemory) && d < pcount;
       uint32_t i = pring + d;
       i = i >= ring_len ? i - ring_len : i;
       i = staged && i < MaxRing ? i : 0u;
-      const uint32_t dp_lds = staged ? flag[i] : 0u;
+      // the flag and the rows in one round of LDS loads (the rows are used
+      // only when the flag says the parent was recomputed)
+      const uint32_t f = flag[i];
+      float4 p0 = rows[0][i], p1 = rows[1][i], p2 = rows[2][i], p3 = rows[3][i];
+      const uint32_t dp_lds = staged ? f : 0u;
       bool dirty = false;
-      float4 p0, p1, p2, p3;
       if (dp_lds) {  // the parent was recomputed by this tile: its rows are in the ring
-        p0 = rows[0][i];
-        p1 = rows[1][i];
-        p2 = rows[2][i];
-        p3 = rows[3][i];
         dirty = true;
       } else if (valid && (!staged || in.dl)) {
         // a kept parent that the tile left unchanged has dirty_world == 0
@@ -307,40 +383,53 @@ __device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32
         load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
         dirty = (in.dl | (staged ? 0u : dp)) != 0;
       }
-      const bool kept = (flags & tileplan::kStaged) != 0;
-      uint32_t o = 0;
-      if (kept) {
-        const uint32_t ring = __builtin_amdgcn_readfirstlane(sg[j].ring);
-        const uint32_t pos = __builtin_amdgcn_readfirstlane(sg[j].pos);
-        o = ring + (threadIdx.x - pos);
+      quad = (h.flags & tileplan::kQuad) != 0;
+      if (dirty) {
+        if (quad) {  // this thread's row only: a quarter of the arithmetic latency
+          w0 = rowmat(pick_row(in), p0, p1, p2, p3);
+          w1 = w0;
+          w2 = w0;
+          w3 = w0;
+        } else {
+          w0 = rowmat(in.l0, p0, p1, p2, p3);
+          w1 = rowmat(in.l1, p0, p1, p2, p3);
+          w2 = rowmat(in.l2, p0, p1, p2, p3);
+          w3 = rowmat(in.l3, p0, p1, p2, p3);
+        }
+      }
+      if (h.flags & tileplan::kStaged) {
+        uint32_t o = h.ring + (in.slot - h.first);
         o = o >= ring_len ? o - ring_len : o;
         o = o < MaxRing ? o : 0u;  // the plan keeps positions below its ring length (at most MaxRing)
+        if (dirty) {
+          if (!quad || in.row == 0) rows[0][o] = w0;
+          if (!quad || in.row == 1) rows[1][o] = w1;
+          if (!quad || in.row == 2) rows[2][o] = w2;
+          if (!quad || in.row == 3) rows[3][o] = w3;
+        }
         flag[o] = dirty ? 1 : 0;
       }
-      if (dirty) {
-        const bool own = in.slot < __builtin_amdgcn_readfirstlane(sg[j].own_end);
-        const uint64_t s = in.slot;
-        const float4 w0 = rowmat(in.l0, p0, p1, p2, p3);
-        const float4 w1 = rowmat(in.l1, p0, p1, p2, p3);
-        const float4 w2 = rowmat(in.l2, p0, p1, p2, p3);
-        const float4 w3 = rowmat(in.l3, p0, p1, p2, p3);
-        if (kept) {
-          rows[0][o] = w0;
-          rows[1][o] = w1;
-          rows[2][o] = w2;
-          rows[3][o] = w3;
-        }
-        if (own) {
-          W[s] = w0;
-          W[ld + s] = w1;
-          W[2 * ld + s] = w2;
-          W[3 * ld + s] = w3;
-          dirty_world[s] = 1;
-        }
-      }
+      store = dirty && in.slot < h.own_end;
     }
-    if (flags & tileplan::kBarrier) lds_barrier();
+    // passes written and read within one wave need no barrier: a wave's LDS
+    // operations execute in order
+    if (h.flags & tileplan::kBarrier) lds_barrier();
+    h = decode_head(hn);
   }
+  // one row plane per thread of a kQuad slot, all four otherwise
+  const uint32_t off = store ? in.slot * 16u : kNoStore;
+  store_row(w0, rw[0], !quad || in.row == 0 ? off : kNoStore);
+  store_row(w1, rw[1], !quad || in.row == 1 ? off : kNoStore);
+  store_row(w2, rw[2], !quad || in.row == 2 ? off : kNoStore);
+  store_row(w3, rw[3], !quad || in.row == 3 ? off : kNoStore);
+  __builtin_amdgcn_raw_buffer_store_b8(static_cast<unsigned char>(1), rdw, store && in.row == 0 ? in.slot : kNoStore,
+                                       0, 0);
+  // the step's loaded registers are consumed on every path: reusing them for a
+  // later step's loads then waits for nothing
+  asm volatile("" ::"v"(in.l0.x), "v"(in.l0.y), "v"(in.l0.z), "v"(in.l0.w), "v"(in.l1.x), "v"(in.l1.y),
+               "v"(in.l1.z), "v"(in.l1.w));
+  asm volatile("" ::"v"(in.l2.x), "v"(in.l2.y), "v"(in.l2.z), "v"(in.l2.w), "v"(in.l3.x), "v"(in.l3.y),
+               "v"(in.l3.z), "v"(in.l3.w), "v"(in.p), "v"(in.dl));
 }
 
 // LDS ring positions per tile: four workgroups of 256 threads per CU (39 KB
@@ -353,8 +442,7 @@ constexpr unsigned tile_max_ring(unsigned threads)
 template <bool HasDirty, unsigned Threads>
 __global__ __launch_bounds__(Threads) void tree_tile_kernel(const float4* __restrict__ L, float4* W, size_t ld,
                                                             const uint32_t* __restrict__ parent,
-                                                            const Seg* __restrict__ segs,
-                                                            const uint32_t* __restrict__ tile_seg, uint32_t ring_len,
+                                                            const Seg* __restrict__ segs, uint32_t ring_len,
                                                             const uint8_t* __restrict__ dirty_local,
                                                             uint8_t* dirty_world)
 {
@@ -362,24 +450,35 @@ __global__ __launch_bounds__(Threads) void tree_tile_kernel(const float4* __rest
   __shared__ float4 rows[4][MaxRing];
   __shared__ uint8_t flag[MaxRing];
   __shared__ Seg sg[kMaxSegs];
-  const uint32_t g0 = tile_seg[blockIdx.x];
-  const uint32_t n = min(tile_seg[blockIdx.x + 1] - g0, kMaxSegs);
-  for (uint32_t i = threadIdx.x; i < n; i += Threads) sg[i] = segs[g0 + i];
+  // the tile's pieces: kMaxSegs per tile, the first one carrying the counts
+  if (threadIdx.x < kMaxSegs) sg[threadIdx.x] = segs[static_cast<size_t>(blockIdx.x) * kMaxSegs + threadIdx.x];
   __syncthreads();
+  const uint32_t n = min(__builtin_amdgcn_readfirstlane(sg[0].tile_pieces), kMaxSegs);
+  const uint32_t steps = min(__builtin_amdgcn_readfirstlane(sg[0].tile_steps), n);
+  if (n == 0) return;  // (the plan gives every tile at least one piece)
+  const uint32_t rec = static_cast<uint32_t>(ld * 16);  // ld <= kMaxBufferSlots (checked by the plan)
+  const __amdgpu_buffer_rsrc_t rw[4] = {__builtin_amdgcn_make_buffer_rsrc(W, 0, rec, kRawBufferDword3),
+                                        __builtin_amdgcn_make_buffer_rsrc(W + ld, 0, rec, kRawBufferDword3),
+                                        __builtin_amdgcn_make_buffer_rsrc(W + 2 * ld, 0, rec, kRawBufferDword3),
+                                        __builtin_amdgcn_make_buffer_rsrc(W + 3 * ld, 0, rec, kRawBufferDword3)};
+  const __amdgpu_buffer_rsrc_t rdw =
+      __builtin_amdgcn_make_buffer_rsrc(dirty_world, 0, static_cast<uint32_t>(ld), kRawBufferDword3);
+  // Steps go in pairs over the register sets A and B; the trip count is fixed
+  // before the loop, so the loop has one exit and the compiler sees every
+  // load of one set consumed before the set is loaded again.
   StepIn A, B;
-  uint32_t a = 0;
-  uint32_t b = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, a, A);
-  for (;;) {
-    // the next step's inputs (past the last step: this step again, unused)
-    const uint32_t c = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, b < n ? b : a, B);
-    run_step<MaxRing>(A, sg, a, b, ring_len, W, ld, dirty_world, rows, flag);
-    if (b >= n) break;
+  uint32_t a = 0;                                                         // first piece of A's step
+  uint32_t b = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, a, A);  // first piece of the next step
+  for (uint32_t k = 0; k + 1 < steps; k += 2) {
+    const uint32_t c = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, b, B);
+    run_step<MaxRing>(A, sg, a, b, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
+    // the step after B's (past the last step: B's step again, unused)
     const uint32_t d = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, c < n ? c : b, A);
-    run_step<MaxRing>(B, sg, b, c, ring_len, W, ld, dirty_world, rows, flag);
-    if (c >= n) break;
+    run_step<MaxRing>(B, sg, b, c, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
     a = c;
     b = d;
   }
+  if (steps & 1u) run_step<MaxRing>(A, sg, a, b, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
 }
 
 // dp::math::invertTranspose (dp/math/Matmnt.h:1171-1201) of W[s]: 2x2 minors of
@@ -466,8 +565,7 @@ struct dpc_tree_plan {
     uint32_t n_tiles = 0;      // kTiles
     uint32_t ring = 0;
     Seg* d_segs = nullptr;
-    uint32_t* d_tile_seg = nullptr;
-  };
+    };
   size_t ld = 0;
   uint32_t threads = 0;  // of a tile workgroup
   uint32_t* d_parent = nullptr;
@@ -477,7 +575,6 @@ struct dpc_tree_plan {
   {
     for (auto& st : steps) {
       if (st.d_segs) (void)hipFree(st.d_segs);
-      if (st.d_tile_seg) (void)hipFree(st.d_tile_seg);
     }
     if (d_parent) (void)hipFree(d_parent);
   }
@@ -496,15 +593,15 @@ void launch_tiles(const dpc_tree_plan& plan, const dpc_tree_plan::Step& st, cons
   switch (plan.threads) {
     case 256:
       hipLaunchKernelGGL((tree_tile_kernel<HasDirty, 256>), grid, dim3(256), 0, s, L, W, plan.ld, plan.d_parent,
-                         st.d_segs, st.d_tile_seg, st.ring, dirty_local, dirty_world);
+                         st.d_segs, st.ring, dirty_local, dirty_world);
       break;
     case 1024:
       hipLaunchKernelGGL((tree_tile_kernel<HasDirty, 1024>), grid, dim3(1024), 0, s, L, W, plan.ld, plan.d_parent,
-                         st.d_segs, st.d_tile_seg, st.ring, dirty_local, dirty_world);
+                         st.d_segs, st.ring, dirty_local, dirty_world);
       break;
     default:
       hipLaunchKernelGGL((tree_tile_kernel<HasDirty, 512>), grid, dim3(512), 0, s, L, W, plan.ld, plan.d_parent,
-                         st.d_segs, st.d_tile_seg, st.ring, dirty_local, dirty_world);
+                         st.d_segs, st.ring, dirty_local, dirty_world);
       break;
   }
 }
@@ -632,9 +729,11 @@ int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_
               "dpc_tree_plan_create: parents");
   // a parent slot at or beyond ld is never read; the planner follows parents of
   // linked passes only, which lie inside the levels
+  // tile runs address the row planes with 32-bit buffer offsets: none beyond kMaxBufferSlots slots
+  const size_t run_passes = ld <= kMaxBufferSlots ? kMaxRunPasses : 1;
   const std::vector<tileplan::Step> sched =
       tileplan::schedule(h_parent_slot, h_level_offsets, n_levels, plan->threads, max_ring, capacity, kLdsNodes,
-                         kMaxLevelsPerSmall, kMaxRunPasses);
+                         kMaxLevelsPerSmall, run_passes);
   for (const auto& ts : sched) {
     dpc_tree_plan::Step st{};
     st.kind = ts.kind;
@@ -657,12 +756,13 @@ int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_
       st.ring = ts.run.ring;
       plan->steps.push_back(st);  // owns the buffers from here on
       auto& ps = plan->steps.back();
-      DPC_HIP_TRY(hipMalloc(&ps.d_segs, ts.run.segs.size() * sizeof(Seg)), "dpc_tree_plan_create: tiles");
-      DPC_HIP_TRY(hipMalloc(&ps.d_tile_seg, ts.run.tile_seg.size() * sizeof(uint32_t)), "dpc_tree_plan_create: tiles");
-      DPC_HIP_TRY(hipMemcpy(ps.d_segs, ts.run.segs.data(), ts.run.segs.size() * sizeof(Seg), hipMemcpyHostToDevice),
-                  "dpc_tree_plan_create: tiles");
-      DPC_HIP_TRY(hipMemcpy(ps.d_tile_seg, ts.run.tile_seg.data(), ts.run.tile_seg.size() * sizeof(uint32_t),
-                            hipMemcpyHostToDevice),
+      // kMaxSegs pieces per tile (check_run bounds the count), so a workgroup finds its tile directly
+      std::vector<Seg> padded(static_cast<size_t>(st.n_tiles) * kMaxSegs, Seg{});
+      for (size_t t = 0; t < st.n_tiles; ++t)
+        std::copy(ts.run.segs.begin() + ts.run.tile_seg[t], ts.run.segs.begin() + ts.run.tile_seg[t + 1],
+                  padded.begin() + t * kMaxSegs);
+      DPC_HIP_TRY(hipMalloc(&ps.d_segs, padded.size() * sizeof(Seg)), "dpc_tree_plan_create: tiles");
+      DPC_HIP_TRY(hipMemcpy(ps.d_segs, padded.data(), padded.size() * sizeof(Seg), hipMemcpyHostToDevice),
                   "dpc_tree_plan_create: tiles");
       plan->tiles += st.n_tiles;
       continue;

@@ -33,6 +33,7 @@
 #pragma once
 
 #include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -42,25 +43,35 @@ namespace dpc {
 namespace tileplan {
 
 // One contiguous piece of one pass of one tile, handled within one step.
-// Plain 32-bit fields; the kernel stages a tile's segments in LDS.
-struct Seg {
+// 32 bytes: the kernel stages a tile's pieces in LDS and reads the first 16
+// bytes (what finding a thread's piece needs) with one LDS load.
+struct alignas(16) Seg {
   uint32_t first;    // first slot of the piece
-  uint32_t count;    // slots in the piece
-  uint32_t pos;      // thread that handles `first` (the others follow)
-  uint32_t ring;     // ring position of `first` (staged passes)
+  uint16_t count;    // slots in the piece
+  uint16_t pos;      // first thread of the piece (a slot takes 1 thread, or 4 with kQuad)
+  uint16_t ring;     // ring position of `first` (kept passes)
+  uint16_t flags;
   uint32_t own_end;  // the tile writes this pass's slots below own_end to memory
   uint32_t pfirst;   // the tile's range of the previous pass: first slot,
-  uint32_t pcount;   //   its length,
-  uint32_t pring;    //   and the ring position of pfirst
-  uint32_t flags;
+  uint16_t pcount;   //   its length,
+  uint16_t pring;    //   and the ring position of pfirst
+  uint32_t tile_pieces;  // in a tile's first piece: the tile's pieces
+  uint32_t tile_steps;   //   and steps (0 in the others)
 };
+static_assert(sizeof(Seg) == 32, "Seg is two 16-byte LDS loads");
 
 enum : uint32_t {
   kStaged = 1,      // results go to the ring: the next pass of the run reads them
-  kBarrier = 2,     // last piece of a staged pass: an LDS barrier follows it
+  kBarrier = 2,     // last piece of a kept pass whose rows other waves read: an LDS barrier follows it
   kStepEnd = 4,     // last piece of its step
   kFromMemory = 8,  // a piece of the run's first pass: parents come from memory
+  kQuad = 16,       // four threads per slot, one matrix row each (passes of a tile with few slots)
 };
+
+// A pass of a tile with at most kQuadMax slots runs four threads per slot:
+// the chain of small passes at the top of a tile then costs a quarter of the
+// arithmetic latency per pass.
+constexpr uint64_t kQuadMax = 16;
 
 struct Limits {
   uint32_t threads;   // slots per step (threads per workgroup)
@@ -126,6 +137,7 @@ inline bool build_run(const uint32_t* par, const uint64_t* off, size_t l, size_t
   out.ring = lim.ring;
   out.tile_seg.assign(1, 0);
   if (r < l + 2 || !first_pass_reads_outside(par, off, l, r)) return false;
+  if (lim.threads == 0 || lim.threads > 0xFFFF || lim.ring == 0 || lim.ring > 0xFFFF) return false;
   const size_t P = r - l;  // passes of the run; pass P-1 (the last) is not kept in LDS
   // first child of every slot of passes l..r-2 (children of a slot are contiguous)
   std::vector<std::vector<uint64_t>> fc(P - 1);
@@ -152,7 +164,7 @@ inline bool build_run(const uint32_t* par, const uint64_t* off, size_t l, size_t
       need = (cnt[k] && k > 0) ? par[end - 1] : UINT64_MAX;
       if (cnt[k]) {
         ++passes;
-        total += cnt[k];
+        total += cnt[k] * (cnt[k] <= kQuadMax ? 4 : 1) + 3;  // threads, and at most 3 left over at a step's end
       }
     }
     for (size_t k = 0; k + 1 < P; ++k) {
@@ -160,41 +172,86 @@ inline bool build_run(const uint32_t* par, const uint64_t* off, size_t l, size_t
       if (k && cnt[k - 1] + cnt[k] > lim.ring) return false;
     }
     if (cnt[P - 1] > lim.free_max) return false;
-    // pieces: at most one per nonempty pass plus one per step boundary
+    // pieces: at most one per nonempty pass plus one per step boundary (threads counted per step)
     return passes + (total + lim.threads - 1) / lim.threads <= lim.max_segs;
   };
   auto emit = [&]() {
     std::vector<uint64_t> o(P, 0);  // ring positions of the kept passes
     for (size_t k = 1; k + 1 < P; ++k) o[k] = (o[k - 1] + cnt[k - 1]) % lim.ring;
+    // Lay the pieces out over the tile's threads, step after step: a slot of
+    // pass k takes width(k) threads and never straddles two steps.
+    struct Piece {
+      size_t k;
+      uint64_t q, n, lane;  // slots [q, q + n) of the tile's range in pass k, from flattened thread `lane`
+      bool step_end;
+    };
+    std::vector<Piece> pcs;
+    const uint64_t T = lim.threads;
+    auto round_up = [&](uint64_t x) { return (x + T - 1) / T * T; };
     uint64_t p = 0;
     for (size_t k = 0; k < P; ++k) {
+      if (!cnt[k]) continue;
+      const uint64_t w = cnt[k] <= kQuadMax ? 4 : 1;
+      if (p % T + w > T) {  // the step has no room for one slot: close it
+        pcs.back().step_end = true;
+        p = round_up(p);
+      }
       for (uint64_t q = 0; q < cnt[k];) {
-        const uint64_t at = p % lim.threads;
-        const uint64_t n = std::min<uint64_t>(cnt[k] - q, lim.threads - at);
-        Seg sg{};
-        sg.first = static_cast<uint32_t>(c[k] + q);
-        sg.count = static_cast<uint32_t>(n);
-        sg.pos = static_cast<uint32_t>(at);
-        sg.ring = k + 1 < P ? static_cast<uint32_t>((o[k] + q) % lim.ring) : 0u;
-        sg.own_end = static_cast<uint32_t>(C[k]);
-        if (k == 0) {
-          sg.flags |= kFromMemory;
-        } else {
-          sg.pfirst = static_cast<uint32_t>(c[k - 1]);
-          sg.pcount = static_cast<uint32_t>(cnt[k - 1]);
-          sg.pring = static_cast<uint32_t>(o[k - 1]);
-        }
-        if (k + 1 < P) {
-          sg.flags |= kStaged;
-          if (q + n == cnt[k]) sg.flags |= kBarrier;
-        }
-        if (at + n == lim.threads) sg.flags |= kStepEnd;
-        out.segs.push_back(sg);
+        const uint64_t n = std::min<uint64_t>(cnt[k] - q, (T - p % T) / w);
+        pcs.push_back(Piece{k, q, n, p, false});
         q += n;
-        p += n;
+        p += n * w;
+        if (p % T == 0 || q < cnt[k]) {  // the step is full
+          pcs.back().step_end = true;
+          p = round_up(p);
+        }
       }
     }
-    out.segs.back().flags |= kStepEnd;
+    pcs.back().step_end = true;
+    // the wave (64-thread block of one step) holding all of pass k, or -1
+    std::vector<int64_t> wave(P, -1);
+    std::vector<uint64_t> lo(P, UINT64_MAX), hi(P, 0);
+    for (const Piece& pc : pcs) {
+      const uint64_t w = cnt[pc.k] <= kQuadMax ? 4 : 1;
+      lo[pc.k] = std::min(lo[pc.k], pc.lane);
+      hi[pc.k] = std::max(hi[pc.k], pc.lane + pc.n * w - 1);
+    }
+    for (size_t k = 0; k < P; ++k)
+      if (cnt[k] && lo[k] / 64 == hi[k] / 64) wave[k] = static_cast<int64_t>(lo[k] / 64);
+    for (const Piece& pc : pcs) {
+      const size_t k = pc.k;
+      Seg sg{};
+      sg.first = static_cast<uint32_t>(c[k] + pc.q);
+      sg.count = static_cast<uint16_t>(pc.n);
+      sg.pos = static_cast<uint16_t>(pc.lane % T);
+      sg.ring = k + 1 < P ? static_cast<uint16_t>((o[k] + pc.q) % lim.ring) : uint16_t(0);
+      sg.own_end = static_cast<uint32_t>(C[k]);
+      uint32_t flags = cnt[k] <= kQuadMax ? kQuad : 0u;
+      if (k == 0) {
+        flags |= kFromMemory;
+      } else {
+        sg.pfirst = static_cast<uint32_t>(c[k - 1]);
+        sg.pcount = static_cast<uint16_t>(cnt[k - 1]);
+        sg.pring = static_cast<uint16_t>(o[k - 1]);
+      }
+      if (k + 1 < P) {
+        flags |= kStaged;
+        // a barrier closes the pass unless its rows are written and read (by
+        // the next pass) within one wave, whose LDS operations execute in
+        // order; a pass that nothing reads needs none
+        const bool readers = cnt[k + 1] > 0;
+        const bool one_wave = wave[k] >= 0 && wave[k] == wave[k + 1];
+        if (pc.q + pc.n == cnt[k] && readers && !one_wave) flags |= kBarrier;
+      }
+      if (pc.step_end) flags |= kStepEnd;
+      sg.flags = static_cast<uint16_t>(flags);
+      out.segs.push_back(sg);
+    }
+    const size_t first = out.tile_seg.back();
+    uint32_t steps = 0;
+    for (size_t j = first; j < out.segs.size(); ++j) steps += (out.segs[j].flags & kStepEnd) ? 1u : 0u;
+    out.segs[first].tile_pieces = static_cast<uint32_t>(out.segs.size() - first);
+    out.segs[first].tile_steps = steps;
     out.tile_seg.push_back(static_cast<uint32_t>(out.segs.size()));
   };
   for (size_t k = 0; k < P; ++k) c[k] = C[k] = off[l + k];
@@ -243,9 +300,10 @@ inline bool build_run(const uint32_t* par, const uint64_t* off, size_t l, size_t
 // Checks every property tree_tile_kernel relies on: pieces inside their
 // pass, the tile's range and the step; ring positions in range; the parent
 // of every slot found at the ring position the tile wrote it to, and not
-// overwritten by the pass that reads it; every slot of the run written by
-// exactly one tile; the first pass reading nothing the run writes.  Returns
-// an empty string, or the first violation.
+// overwritten by the pass that reads it; a kept pass without a closing
+// barrier written and read (by the next pass) within one wave of one step;
+// every slot of the run written by exactly one tile; the first pass reading
+// nothing the run writes.  Returns an empty string, or the first violation.
 inline std::string check_run(const uint32_t* par, const uint64_t* off, const Limits& lim, const Run& run)
 {
   char msg[256];
@@ -265,32 +323,52 @@ inline std::string check_run(const uint32_t* par, const uint64_t* off, const Lim
     const uint32_t g0 = run.tile_seg[t], g1 = run.tile_seg[t + 1];
     if (g1 <= g0) return bad("empty tile", t, 0);
     if (g1 - g0 > lim.max_segs) return bad("too many pieces", t, 0);
+    if (!(run.segs[g1 - 1].flags & kStepEnd)) return bad("tile does not end a step", t, g1 - 1);
+    uint32_t steps = 0;
+    for (uint32_t j = g0; j < g1; ++j) steps += (run.segs[j].flags & kStepEnd) ? 1u : 0u;
+    if (run.segs[g0].tile_pieces != g1 - g0 || run.segs[g0].tile_steps != steps)
+      return bad("tile piece or step count wrong", t, g0);
     std::fill(ring_slot.begin(), ring_slot.end(), -1);
     std::fill(ring_pass.begin(), ring_pass.end(), -1);
+    // pass index and wave block (step * threads + position) / 64 of every piece
+    std::vector<size_t> pass_of(g1 - g0);
+    std::vector<int64_t> wave_lo(g1 - g0), wave_hi(g1 - g0);
     size_t pass = l;
-    uint64_t next_pos = 0;         // position the next piece must start at within its step
+    uint64_t step = 0, next_pos = 0;
     uint64_t pass_first = 0, pass_count = 0, pass_ring = 0, pass_own = 0;
     uint64_t prev_first = 0, prev_count = 0, prev_ring = 0;
     bool have_pass = false;
     std::vector<uint32_t> written;  // ring positions written by the current pass
+    auto commit = [&]() {           // the current pass's rows become readable
+      for (size_t i = 0; i < written.size(); ++i) {
+        ring_slot[written[i]] = static_cast<int64_t>(pass_first + i);
+        ring_pass[written[i]] = static_cast<int64_t>(pass);
+      }
+      written.clear();
+    };
     for (uint32_t j = g0; j < g1; ++j) {
       const Seg& s = run.segs[j];
+      const uint64_t lanes = static_cast<uint64_t>(s.count) * ((s.flags & kQuad) ? 4 : 1);
       if (s.count == 0) return bad("empty piece", t, j);
       if (s.pos != next_pos) return bad("pieces of a step not contiguous", t, j);
-      if (static_cast<uint64_t>(s.pos) + s.count > lim.threads) return bad("piece beyond the step", t, j);
-      next_pos = (s.flags & kStepEnd) ? 0 : s.pos + s.count;
+      if (s.pos + lanes > lim.threads) return bad("piece beyond the step", t, j);
+      wave_lo[j - g0] = static_cast<int64_t>((step * lim.threads + s.pos) / 64);
+      wave_hi[j - g0] = static_cast<int64_t>((step * lim.threads + s.pos + lanes - 1) / 64);
+      if (s.flags & kStepEnd) {
+        next_pos = 0;
+        ++step;
+      } else {
+        next_pos = s.pos + lanes;
+      }
       // which pass: continue the current one or start the next
       const bool cont = have_pass && s.first == pass_first + pass_count;
-      bool in_cur = have_pass && s.first >= off[pass] && s.first < off[pass + 1] && cont;
+      const bool in_cur = cont && s.first >= off[pass] && s.first < off[pass + 1];
       if (!in_cur) {
         if (have_pass) {
-          // the previous pass is complete: its last piece carried the barrier if kept
-          const Seg& last = run.segs[j - 1];
-          if (pass + 1 < r && !(last.flags & kBarrier)) return bad("kept pass without a closing barrier", t, j - 1);
+          commit();
           prev_first = pass_first;
           prev_count = pass_count;
           prev_ring = pass_ring;
-          written.clear();
           ++pass;
         }
         if (pass >= r) return bad("more passes than the run", t, j);
@@ -300,6 +378,7 @@ inline std::string check_run(const uint32_t* par, const uint64_t* off, const Lim
         pass_ring = s.ring;
         pass_own = s.own_end;
       }
+      pass_of[j - g0] = pass;
       if (s.first < off[pass] || static_cast<uint64_t>(s.first) + s.count > off[pass + 1])
         return bad("piece outside its pass", t, j);
       if (s.own_end != pass_own) return bad("ownership changes within a pass", t, j);
@@ -335,14 +414,22 @@ inline std::string check_run(const uint32_t* par, const uint64_t* off, const Lim
       }
       pass_count += s.count;
       if (kept && pass_count > lim.ring) return bad("pass longer than the ring", t, j);
-      // the pass's rows are visible to the next pass after its closing barrier
-      if (kept && (s.flags & kBarrier))
-        for (size_t i = 0; i < written.size(); ++i) {
-          ring_slot[written[i]] = static_cast<int64_t>(pass_first + i);
-          ring_pass[written[i]] = static_cast<int64_t>(pass);
-        }
     }
-    if (!(run.segs[g1 - 1].flags & kStepEnd)) return bad("tile does not end a step", t, g1 - 1);
+    // a kept pass whose last piece has no barrier: it and the next pass lie in one wave
+    for (uint32_t j = g0; j < g1; ++j) {
+      const size_t k = pass_of[j - g0];
+      const bool last_of_pass = j + 1 == g1 || pass_of[j + 1 - g0] != k;
+      if (!last_of_pass || k + 1 >= r || (run.segs[j].flags & kBarrier)) continue;
+      int64_t lo = INT64_MAX, hi = INT64_MIN;
+      bool read = false;  // the tile has pieces of pass k + 1
+      for (uint32_t i = g0; i < g1; ++i)
+        if (pass_of[i - g0] == k || pass_of[i - g0] == k + 1) {
+          read = read || pass_of[i - g0] == k + 1;
+          lo = std::min(lo, wave_lo[i - g0]);
+          hi = std::max(hi, wave_hi[i - g0]);
+        }
+      if (read && lo != hi) return bad("pass without a barrier is read by another wave", t, j);
+    }
   }
   for (uint64_t s = base; s < end; ++s)
     if (!owned[s - base]) {

@@ -104,18 +104,23 @@ static void emulate_run(const Layout& lay, const tp::Run& run, uint32_t threads,
         if (end || b >= n) break;
       }
       // each thread's piece and slot (load_step)
-      std::vector<uint32_t> mine(threads, UINT32_MAX), slot(threads, 0);
-      for (uint32_t j = a; j < b; ++j)
+      // (a kQuad piece gives each slot four threads, one matrix row each: the
+      // emulation computes the slot once, at its first thread)
+      std::vector<uint32_t> mine(threads, UINT32_MAX), slot(threads, 0), row(threads, 0);
+      for (uint32_t j = a; j < b; ++j) {
+        const uint32_t width = (sg[j].flags & tp::kQuad) ? 4 : 1;
         for (uint32_t x = 0; x < threads; ++x)
-          if (x - sg[j].pos < sg[j].count) {
+          if (x - sg[j].pos < sg[j].count * width) {
             EXPECT(mine[x] == UINT32_MAX, "thread %u has two pieces in a step", x);
             mine[x] = j;
-            slot[x] = sg[j].first + (x - sg[j].pos);
+            slot[x] = sg[j].first + (x - sg[j].pos) / width;
+            row[x] = (x - sg[j].pos) % width;
           }
+      }
       for (uint32_t j = a; j < b; ++j) {
         const tp::Seg& s = sg[j];
         for (uint32_t x = 0; x < threads; ++x) {
-          if (mine[x] != j) continue;
+          if (mine[x] != j || row[x] != 0) continue;
           const uint64_t sl = slot[x];
           const uint64_t p = lay.par[sl];
           const bool valid = p < lay.ld;
@@ -139,7 +144,7 @@ static void emulate_run(const Layout& lay, const tp::Run& run, uint32_t threads,
           const bool kept = (s.flags & tp::kStaged) != 0;
           uint32_t o = 0;
           if (kept) {
-            o = s.ring + (x - s.pos);
+            o = s.ring + static_cast<uint32_t>(sl - s.first);
             o = o >= R ? o - R : o;
             EXPECT(o < R, "ring position %u >= %u", o, R);
             if (o >= R) continue;
