@@ -255,6 +255,8 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
                             frusta=int(reqs[i].vps.shape[0]), delta=int(reqs[i].delta_idx.size)) for i in worst]
         runs.append(dict(_issue_total=st_now["issue_us"], worst_queue=worst_queue,
                          scheduler_issue_max_us_so_far=st_now["issue_max_us"], scheduler_issue_frac=issue_us * 1e-6 / span_s,
+                         scheduler_issue_part_max_us_so_far=dict(zip(("delta_copies", "delta_scatters", "descriptors", "launch", "masks"),
+                                                                     [round(x, 1) for x in st_now["issue_part_max_us"]])),
                          pcie_h2d_gbs=delta_bytes / span_s / 1e9, pcie_d2h_gbs=mask_bytes / span_s / 1e9,
                          offered_rate=rate, requests=n_req, preroll_requests=n_pre,
                          achieved_submit_rate=n_req / ((t_submitted - start - shift) * 1e-6),

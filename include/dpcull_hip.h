@@ -308,6 +308,10 @@ typedef struct dpc_server_stats {
   double busy_us;            /* sum over batches of launch-to-completion time */
   double issue_us;           /* scheduler-thread time spent issuing batches (copies, launches) */
   double issue_max_us;       /* the longest single batch issue */
+  /* the longest time one batch spent in each part of its issue: [0] delta
+   * copies, [1] delta scatter launches, [2] descriptor copy, [3] cull launch,
+   * [4] mask copies and completion event */
+  double issue_part_max_us[5];
 } dpc_server_stats;
 
 int dpc_server_create(const dpc_server_config* cfg, dpc_server** server);

@@ -227,13 +227,18 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   const uint64_t end = masks_off + 8 * mask_word;
   int rc = DPC_OK;
   if (end > slot.cap) rc = fail(DPC_ERR_OVER_BUDGET, "batch larger than its arena");
+  double t_part[6];
+  double copy_us = 0.0;
+  t_part[0] = now();
   // deltas: copy to the device and scatter into the scenes' matrices
   for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
     const Request& r = *pick[k];
     if (!r.n_delta) continue;
+    const double tc = now();
     e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(slot.d_arena + mat_off[k], r.dmats, 64ull * r.n_delta, hipMemcpyHostToDevice, s);
+    copy_us += now() - tc;
     if (e != hipSuccess) {
       rc = hip_fail(e, "delta upload");
       break;
@@ -242,15 +247,19 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
                               reinterpret_cast<const uint32_t*>(slot.d_arena + idx_off[k]),
                               reinterpret_cast<const float*>(slot.d_arena + mat_off[k]), r.n_delta, s);
   }
+  t_part[2] = now();
+  t_part[1] = t_part[0] + copy_us;  // parts 0 and 1: copies, then the scatter launches (interleaved in time)
   if (rc == DPC_OK) {
     e = hipMemcpyAsync(slot.d_arena, slot.h_arena, desc_end, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(e, "descriptor upload");
   }
+  t_part[3] = now();
   if (rc == DPC_OK) {
     rc = dpc::launch_serve(slot.d_arena + entries_off, static_cast<uint32_t>(pick.size()), cost,
                            slot.d_arena + pairs_off, d_scene_table, reinterpret_cast<const float*>(slot.d_arena + vps_off),
                            reinterpret_cast<uint64_t*>(slot.d_arena + masks_off), K, cfg.waves_per_cu, s);
   }
+  t_part[4] = now();
   // each request's masks go straight to the caller's buffer
   for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
     const Request& r = *pick[k];
@@ -262,9 +271,11 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
     if (e != hipSuccess) rc = hip_fail(e, "completion event");
   }
   const double t_issue1 = now();
+  t_part[5] = t_issue1;
   lock.lock();
   stats.issue_us += t_issue1 - t_issue0;
   stats.issue_max_us = std::max(stats.issue_max_us, t_issue1 - t_issue0);
+  for (int q = 0; q < 5; ++q) stats.issue_part_max_us[q] = std::max(stats.issue_part_max_us[q], t_part[q + 1] - t_part[q]);
   slot.reqs = pick;
   slot.busy = true;
   slot.t_launch = t_admit;

@@ -138,7 +138,11 @@ class Server:
     def stats(self) -> dict:
         st = _lib.ServerStats()
         check(self.lib.dpc_server_stats_get(self.handle, C.byref(st)))
-        return {name: getattr(st, name) for name, _ in _lib.ServerStats._fields_}
+        out = {}
+        for name, _ in _lib.ServerStats._fields_:
+            v = getattr(st, name)
+            out[name] = list(v) if isinstance(v, C.Array) else v
+        return out
 
     def now_us(self) -> float:
         return float(self.lib.dpc_server_now_us(self.handle))
