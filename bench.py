@@ -375,7 +375,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}); "
-              "see the sharding design in DESIGN.md", file=sys.stderr)
+              "culling would shard by group and propagation by subtree, with no exchange", file=sys.stderr)
         sys.exit(2)
 
     from pipeline_amd import _lib, scenes
