@@ -312,6 +312,7 @@ typedef struct dpc_server_stats {
    * copies, [1] delta scatter launches, [2] descriptor copy, [3] cull launch,
    * [4] mask copies and completion event */
   double issue_part_max_us[5];
+  uint64_t delta_direct;     /* requests whose delta the device read straight from pinned host memory */
 } dpc_server_stats;
 
 int dpc_server_create(const dpc_server_config* cfg, dpc_server** server);
@@ -326,8 +327,10 @@ int dpc_server_remove_scene(dpc_server* server, uint32_t scene_id);
  * indices) and delta_mats (n_delta x 16) are read while the request is being
  * launched; masks_out (at least n_frusta x ceil(n_objects / 64) words,
  * frustum-major, the dp::util::BitArray layout) receives the result.  All four
- * are borrowed until dpc_server_wait returns for the request; pinned host memory
- * (dpc_host_alloc) makes the copies DMA transfers. */
+ * are borrowed until dpc_server_wait returns for the request.  Delta arrays that
+ * each lie within one pinned host allocation (dpc_host_alloc) are read by the
+ * device directly, with no copy; others are copied first.  A masks_out in pinned
+ * memory receives a DMA transfer. */
 int dpc_server_submit(dpc_server* server, uint32_t scene_id, const float* vps, uint32_t n_frusta,
                       const uint32_t* delta_idx, const float* delta_mats, uint32_t n_delta, uint64_t* masks_out,
                       uint64_t mask_capacity_words, uint64_t* request_id);

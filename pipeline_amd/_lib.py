@@ -56,7 +56,8 @@ class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_uint64), ("batches", C.c_uint64), ("tests", C.c_uint64),
                 ("delta_matrices", C.c_uint64), ("scene_bytes", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
                 ("busy_us", C.c_double), ("issue_us", C.c_double),
-                ("issue_max_us", C.c_double), ("issue_part_max_us", C.c_double * 5)]
+                ("issue_max_us", C.c_double), ("issue_part_max_us", C.c_double * 5),
+                ("delta_direct", C.c_uint64)]
 
 
 class CullConfig(C.Structure):

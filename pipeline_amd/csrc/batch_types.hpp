@@ -32,4 +32,14 @@ int launch_serve(const void* d_entries, uint32_t n_entries, uint64_t total_cost,
                  const void* d_scenes, const float* d_vps, uint64_t* d_masks, uint32_t objects_per_lane,
                  uint32_t waves_per_cu, void* stream);
 
+// Scatters count delta matrices (16 floats each, row-major) into row planes,
+// reading the indices and matrices from pinned host memory (device-visible
+// pointers) with system-coherent loads; no copy is needed before it.  At most
+// kMaxHostScatter matrices into at most kMaxHostScatterLd slots (32-bit byte
+// offsets).
+constexpr uint32_t kMaxHostScatter = 1u << 25;
+constexpr uint32_t kMaxHostScatterLd = (1u << 26) - 1;
+int launch_scatter_host(float* d_planes, size_t ld, const uint32_t* idx, const float* src, size_t count,
+                        void* stream);
+
 }  // namespace dpc

@@ -10,6 +10,7 @@
 
 #include "cull_math.hpp"
 #include "internal.hpp"
+#include "batch_types.hpp"
 
 #pragma clang fp contract(off)
 
@@ -130,6 +131,34 @@ __global__ void scatter_kernel(float4* __restrict__ planes, size_t ld, const uin
   const size_t k = t >> 2, r = t & 3;
   const size_t s = idx[k];
   if (s < ld) planes[r * ld + s] = src[k * 4 + r];
+}
+
+// The same scatter with the delta read straight from pinned host memory:
+// system-coherent 16-byte buffer loads (sc0 sc1: they see the host's writes
+// and are not kept in device caches), so no copy precedes it.  A small grid
+// strides over the rows: enough loads in flight to fill PCIe, few waves
+// holding CUs while they wait on it.
+constexpr int kSystemCoherent = 17;  // buffer load cache policy: sc0 | sc1
+constexpr int kRawBuffer = 0x00020000;
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void scatter_host_kernel(float4* __restrict__ planes, uint32_t ld, const uint32_t* idx,
+                                                           const float* src, uint32_t count)
+{
+  const __amdgpu_buffer_rsrc_t ri =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t*>(idx), 0, count * 4u, kRawBuffer);
+  const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(src), 0, count * 64u, kRawBuffer);
+  // the four row planes as one buffer (ld <= kMaxHostScatterLd keeps it below 4 GB)
+  const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(planes, 0, ld * 64u, kRawBuffer);
+  const uint32_t rows = count * 4u;
+  for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < rows; t += gridDim.x * 256u) {
+    const uint32_t k = t >> 2, r = t & 3u;
+    // both loads issue before either is waited for; the store is issued on
+    // every path (an index outside the planes gets an offset the hardware drops)
+    const uint32_t s = __builtin_amdgcn_raw_buffer_load_b32(ri, k * 4u, 0, kSystemCoherent);
+    const u32x4_t v = __builtin_amdgcn_raw_buffer_load_b128(rs, t * 16u, 0, kSystemCoherent);
+    __builtin_amdgcn_raw_buffer_store_b128(v, rp, s < ld ? (r * ld + s) * 16u : 0xFFFFFFF0u, 0, 0);
+  }
 }
 
 __global__ void gather_u32_kernel(const uint32_t* __restrict__ table, size_t table_len,
@@ -494,6 +523,28 @@ int dpc_matrices_scatter(float* d_planes, size_t ld, const uint32_t* d_idx, cons
                      reinterpret_cast<float4*>(d_planes), ld, d_idx, reinterpret_cast<const float4*>(d_src), count);
   return check_launch("scatter_kernel");
 }
+
+}  // extern "C"
+
+namespace dpc {
+
+int launch_scatter_host(float* d_planes, size_t ld, const uint32_t* idx, const float* src, size_t count,
+                        void* stream)
+{
+  if (count == 0) return DPC_OK;
+  if (count > kMaxHostScatter || ld > kMaxHostScatterLd)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "launch_scatter_host: %zu matrices into %zu slots exceed %u into %u",
+                     count, ld, kMaxHostScatter, kMaxHostScatterLd);
+  // 64 workgroups (256 CUs hold 4x more): ~16k rows of 16 B in flight, well above PCIe's bandwidth-latency product
+  const unsigned grid = std::min<unsigned>(grid_for(count * 4, 256), 64u);
+  hipLaunchKernelGGL(scatter_host_kernel, dim3(grid), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<float4*>(d_planes), static_cast<uint32_t>(ld), idx, src, static_cast<uint32_t>(count));
+  return check_launch("scatter_host_kernel");
+}
+
+}  // namespace dpc
+
+extern "C" {
 
 int dpc_gather_u32(const uint32_t* d_table, size_t table_len, const uint32_t* d_idx, size_t n, uint32_t* d_out,
                    void* stream)

@@ -66,6 +66,10 @@ struct Request {
   const uint32_t* didx = nullptr;
   const float* dmats = nullptr;
   uint32_t n_delta = 0;
+  // both delta arrays lie in pinned host memory: the device reads them through
+  // these pointers, with no copy
+  const uint32_t* didx_dev = nullptr;
+  const float* dmats_dev = nullptr;
   uint64_t* masks_out = nullptr;
   uint64_t words = 0;       // mask words per frustum
   uint64_t dev_bytes = 0;   // arena bytes this request needs
@@ -125,6 +129,26 @@ struct dpc_server {
 namespace {
 
 int fail(int code, const std::string& msg) { return dpc::set_error(code, "%s", msg.c_str()); }
+
+// The device-visible address of [p, p + bytes) when the whole range lies in
+// one pinned host allocation (hipHostMalloc), else nullptr.
+const void* pinned_range(const void* p, size_t bytes)
+{
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess || a.type != hipMemoryTypeHost || !a.devicePointer) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  hipDeviceptr_t base = nullptr;
+  size_t size = 0;
+  if (hipMemGetAddressRange(&base, &size, reinterpret_cast<hipDeviceptr_t>(const_cast<void*>(p))) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  const uintptr_t b = reinterpret_cast<uintptr_t>(base), q = reinterpret_cast<uintptr_t>(p);
+  if (q < b || bytes > size || q - b > size - bytes) return nullptr;
+  return a.devicePointer;
+}
 
 int hip_fail(hipError_t e, const char* what)
 {
@@ -191,10 +215,11 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   std::vector<uint64_t> idx_off(pick.size()), mat_off(pick.size());
   for (size_t k = 0; k < pick.size(); ++k) {
     const Request& r = *pick[k];
+    const uint64_t staged = r.didx_dev ? 0 : r.n_delta;
     idx_off[k] = off;
-    off = align_up(off + 4ull * r.n_delta, 256);
+    off = align_up(off + 4ull * staged, 256);
     mat_off[k] = off;
-    off = align_up(off + 64ull * r.n_delta, 256);
+    off = align_up(off + 64ull * staged, 256);
   }
   const uint64_t masks_off = off;
   auto* entries = reinterpret_cast<dpc::ServeEntry*>(slot.h_arena + entries_off);
@@ -234,6 +259,10 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
     const Request& r = *pick[k];
     if (!r.n_delta) continue;
+    if (r.didx_dev) {  // pinned: the scatter reads the caller's arrays directly
+      rc = dpc::launch_scatter_host(snapshot[k].planes, snapshot[k].n_mat, r.didx_dev, r.dmats_dev, r.n_delta, s);
+      continue;
+    }
     const double tc = now();
     e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
     if (e == hipSuccess)
@@ -297,7 +326,10 @@ void dpc_server::complete(StreamSlot& slot)
   const double t = now();
   for (size_t k = 0; k < slot.reqs.size(); ++k) {
     Request& r = *slot.reqs[k];
-    if (r.status == DPC_OK) stats.delta_matrices += r.n_delta;
+    if (r.status == DPC_OK) {
+      stats.delta_matrices += r.n_delta;
+      if (r.n_delta && r.didx_dev) stats.delta_direct++;
+    }
     r.t_done = t;
     r.done = true;
     Scene& sc = scenes[r.scene];
@@ -547,16 +579,29 @@ int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32
   r->dmats = dmats;
   r->n_delta = n_delta;
   r->masks_out = masks_out;
+  if (n_delta && n_delta <= dpc::kMaxHostScatter) {
+    const void* di = pinned_range(didx, 4ull * n_delta);
+    const void* dm = di ? pinned_range(dmats, 64ull * n_delta) : nullptr;
+    if (di && dm) {
+      r->didx_dev = static_cast<const uint32_t*>(di);
+      r->dmats_dev = static_cast<const float*>(dm);
+    }
+  }
   std::lock_guard<std::mutex> g(s->mu);
   if (scene_id >= s->scenes.size() || !s->scenes[scene_id].live) return fail(DPC_ERR_NOT_FOUND, "no such scene");
   const Scene& sc = s->scenes[scene_id];
+  if (sc.n_mat > dpc::kMaxHostScatterLd) {  // the direct scatter addresses the planes with 32-bit offsets
+    r->didx_dev = nullptr;
+    r->dmats_dev = nullptr;
+  }
   // delta indices are checked here, on the host, so the scatter never writes outside the scene
   for (uint32_t k = 0; k < n_delta; ++k)
     if (didx[k] >= sc.n_mat) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: delta index out of range");
   r->words = (sc.n + 63) / 64;
   if (mask_capacity_words < r->words * n_frusta)
     return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: the mask buffer is smaller than n_frusta x ceil(n/64) words");
-  r->dev_bytes = align_up(4ull * n_delta, 256) + align_up(64ull * n_delta, 256) + sizeof(dpc::ServeEntry) +
+  const uint64_t staged_delta = r->didx_dev ? 0 : n_delta;  // a pinned delta takes no arena space
+  r->dev_bytes = align_up(4ull * staged_delta, 256) + align_up(64ull * staged_delta, 256) + sizeof(dpc::ServeEntry) +
                  (sizeof(dpc::KPair) + 64) * n_frusta + 8 * r->words * n_frusta + 1024;
   if (r->dev_bytes > s->slots[0].cap)
     return fail(DPC_ERR_OVER_BUDGET, "dpc_server_submit: the request alone exceeds the in-flight budget");

@@ -99,6 +99,45 @@ def test_serve_mix_equals_alone_and_oracle(orc):
             assert np.array_equal(m, mix[i][1]), f"scene {s} request {i}: alone != mix"
 
 
+def test_serve_pinned_deltas_read_directly(orc):
+    """Deltas whose two arrays lie in pinned host memory are read by the device
+    directly (no copy); mixed in one batch with copied deltas, every request's
+    masks still equal the oracle's replay."""
+    scenes = S.serving_scenes(seed=11, n_scenes=6, size_range=(3_000, 120_000))
+    reqs = S.serving_requests(scenes, seed=12, n_requests=48, rate_per_s=0.0)
+    pinned, mixed, direct = [], [], 0
+    for i, r in enumerate(reqs):
+        if r.delta_idx.size and i % 3 != 2:
+            pi = PinnedArray(r.delta_idx.shape, np.uint32)
+            pi.array[:] = r.delta_idx
+            pinned.append(pi)
+            if i % 3 == 0:  # both arrays pinned: read directly
+                pm = PinnedArray(r.delta_mats.shape, np.float32)
+                pm.array[:] = r.delta_mats
+                pinned.append(pm)
+                mixed.append(S.ServingRequest(r.scene, r.vps, pi.array, pm.array, r.arrival_us))
+                direct += 1
+            else:  # only the indices pinned: copied
+                mixed.append(S.ServingRequest(r.scene, r.vps, pi.array, r.delta_mats, r.arrival_us))
+        else:
+            mixed.append(r)
+    mix, stats = _run_mix(scenes, mixed)
+    assert all(info.status == 0 for info, _ in mix)
+    assert stats["delta_direct"] == direct and direct > 0
+    assert max(info.batch_requests for info, _ in mix) > 1
+    rep = Replay(orc, scenes)
+    bad = 0
+    for r, (info, m) in zip(mixed, mix):
+        want = rep.apply_and_cull(r)
+        n = scenes[r.scene].n
+        words = (n + 63) // 64
+        for f in range(r.vps.shape[0]):
+            bad += mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
+    for p in pinned:
+        p.free()
+    assert bad == 0, f"{bad} visibility bits differ from the oracle"
+
+
 def test_serve_budget_admission(orc):
     """Requests are admitted while the in-flight arena of a stream has room: two
     whole-scene deltas never share a batch, small requests do."""
