@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X scene-update path: object-frustum tests/sec on one
-GPU, batched (one persistent hipGraph-captured launch for all frusta) versus the
-plain per-view launch (one launch per frustum), with 0 visibility-bit mismatches.
+GPU, batched (one persistent launch for all frusta; its hipGraph replay is timed
+beside it) versus the plain per-view launch (one launch per frustum), with 0
+visibility-bit mismatches.
 
 Workload (synthetic, seeded, FP32): 2,097,152 objects in 64 groups of
 log-uniform size, 64 frusta per launch (4 shadow-cascade ortho boxes, 6 cubemap
@@ -12,8 +13,8 @@ timed region.
 Also measured in the same run and reported beside the headline:
   flat      1,048,576 objects x 1 frustum (HBM-bound), GB/s over four
             scene copies so the 256 MB last-level cache does not hold the input
-  tree      1,000,000-node transform tree propagated level by level, then a
-            cull of its ~2.4M leaf objects
+  tree      1,000,000-node transform tree propagated by its launch schedule (one
+            tile launch), then a cull of its ~2.4M leaf objects
   serving   16 resident scenes, Poisson requests at 2k-20k req/s (1-32 frusta and a
             1-10% matrix delta each), p50/p99 latency and tests/s per rate
   cpu       the CPU oracle (a restatement of dp::culling::cpu::ManagerImpl::cull)
@@ -76,6 +77,16 @@ def cpu_model() -> str:
 def log(msg):
     sys.stderr.write(f"[bench] {msg}\n")
     sys.stderr.flush()
+
+
+def prewarm(lib, stream, issue, seconds=0.3):
+    """Untimed work for `seconds` before any measurement: the GPU's clocks ramp
+    up over tens of milliseconds, longer than a few warm-up steps take."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(32):
+            issue()
+        stream.sync()
 
 
 def run_timed(dev, lib, stream, steps, warmup, issue):
@@ -418,11 +429,14 @@ def main():
     def issue_per_view():
         plan.cull_per_view(bs.vps, masks_pv, stream.handle)
 
+    prewarm(lib, stream, issue_batch)
+    # the step: one launch of the batched kernel, K of them back to back; the
+    # event pair around them is the kernel's time base for the roofline
+    wall_b, k_mean_b, k_med_b = run_timed(dev, lib, stream, args.steps, args.warmup, issue_batch)
+    # the same step replayed from a captured hipGraph, reported beside it (on
+    # this runtime a replay costs more per step than a direct launch)
     graph = dev.Graph(stream, issue_batch)
-    # the step: one replay of the captured batched launch
-    wall_b, _, _ = run_timed(dev, lib, stream, args.steps, args.warmup, lambda: graph.launch(stream))
-    # the kernel: back-to-back direct launches between one event pair (roofline time base)
-    _, k_mean_b, k_med_b = run_timed(dev, lib, stream, args.steps, args.warmup, issue_batch)
+    wall_g, _, _ = run_timed(dev, lib, stream, args.steps, args.warmup, lambda: graph.launch(stream))
     wall_v, k_mean_v, k_med_v = run_timed(dev, lib, stream, args.steps, args.warmup, issue_per_view)
     batched = masks.download(stream.handle)
     per_view = masks_pv.download(stream.handle)
@@ -449,10 +463,12 @@ def main():
     value = tests / (wall_b / args.steps)
     per_view_value = tests / (wall_v / args.steps)
     achieved_tops = ops / (k_mean_b * 1e-3) / 1e12
-    log(f"batched: {k_mean_b * 1e3:.1f} us/launch (event), {wall_b / args.steps * 1e6:.1f} us/step wall, "
+    log(f"batched: {k_mean_b * 1e3:.1f} us/launch (event), {wall_b / args.steps * 1e6:.1f} us/step wall "
+        f"({wall_g / args.steps * 1e6:.1f} us/step as graph replays), "
         f"{value / 1e9:.2f} G tests/s; per-view {wall_v / args.steps * 1e6:.1f} us/step, "
         f"{per_view_value / 1e9:.2f} G tests/s; mismatches vs per-view {mism_pv}, vs oracle sample {mism_orc}")
     result["batched"] = dict(value=value, ms_per_step=wall_b / args.steps * 1e3, kernel_us=k_mean_b * 1e3,
+                             graph_replay_us_per_step=wall_g / args.steps * 1e6,
                              kernel_us_median=k_med_b * 1e3, tests=tests, objects_referenced=referenced,
                              algorithmic_ops=ops, algorithmic_bytes=hbm_bytes,
                              achieved_tops=achieved_tops, achieved_gbs=hbm_bytes / (k_mean_b * 1e-3) / 1e9)
@@ -529,6 +545,7 @@ def main():
             _lib.check(lib.dpc_cull(s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr, s.ld, vp16,
                                     fmask[k].ptr, stream.handle))
 
+        prewarm(lib, stream, issue_flat)  # (the CPU baseline before this left the GPU idle)
         wall_f, k_mean_f, k_med_f = run_timed(dev, lib, stream, args.steps, args.warmup, issue_flat)
         l4f, e4f = scenes.extents(fs.lower, fs.upper)
         want = orc.cull_group(l4f, e4f, np.arange(fs.n), fs.matrices, fs.vp)
@@ -573,9 +590,11 @@ def main():
             _lib.check(lib.dpc_cull(objs.box_a.ptr, objs.box_b.ptr, objs.xform.ptr, objs.n_objects, d_world.ptr,
                                     n_slots, tvp, tmask.ptr, stream.handle))
 
+        t_steps = max(args.steps // 4, 5)
+        prewarm(lib, stream, issue_tree)
+        wall_t, k_mean_t, _ = run_timed(dev, lib, stream, t_steps, args.warmup, issue_tree)
         tgraph = dev.Graph(stream, issue_tree)
-        wall_t, k_mean_t, _ = run_timed(dev, lib, stream, max(args.steps // 4, 5), args.warmup,
-                                        lambda: tgraph.launch(stream))
+        wall_tg, _, _ = run_timed(dev, lib, stream, t_steps, args.warmup, lambda: tgraph.launch(stream))
         tgraph.destroy()
         # parity of this run's results: world matrices vs the FP32 oracle (ulp) and
         # an FP64 evaluation of the same recurrence; the cull mask vs the oracle
@@ -605,13 +624,15 @@ def main():
         prop_bytes = n_nodes * (64 + 64 + 4) + 64 * (n_slots - int(np.diff(ts.level_offsets)[-1]))
         cull_bytes = m_obj * (24 + 4) + 64 * len(np.unique(ts.obj_slot)) + m_obj / 8
         result["tree"] = dict(nodes=n_nodes, levels=len(ts.level_offsets) - 1, objects=m_obj,
-                              us_per_step=wall_t / max(args.steps // 4, 5) * 1e6, step_event_us=k_mean_t * 1e3,
+                              us_per_step=wall_t / t_steps * 1e6, step_event_us=k_mean_t * 1e3,
+                              graph_replay_us_per_step=wall_tg / t_steps * 1e6,
                               achieved_gbs=(prop_bytes + cull_bytes) / (k_mean_t * 1e-3) / 1e9,
-                              tests_per_s=m_obj / (wall_t / max(args.steps // 4, 5)),
+                              tests_per_s=m_obj / (wall_t / t_steps),
                               world_max_ulp_vs_oracle=max_ulp, world_max_rel_err_vs_fp64=max_rel,
                               cull_mismatches=mism_t,
                               propagation_launches=t_launches.value, tile_runs=t_runs.value, tiles=t_tiles.value)
-        log(f"tree: {result['tree']['us_per_step']:.1f} us/step (propagate + cull)")
+        log(f"tree: {result['tree']['us_per_step']:.1f} us/step (propagate + cull; "
+            f"{result['tree']['graph_replay_us_per_step']:.1f} us as graph replays)")
         for d in (d_local, d_world, d_dirty, tmask):
             d.free()
         objs.free()
