@@ -221,7 +221,7 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
  * that layout. */
 typedef struct dpc_tree_plan dpc_tree_plan;
 int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
-                         uint32_t tile_slots, dpc_tree_plan** plan);  /* tile_slots: 256, 512, 1024 or 0 (512) */
+                         uint32_t tile_slots, dpc_tree_plan** plan);  /* tile_slots: 256, 512, 1024 or 0 (1024) */
 int dpc_tree_plan_destroy(dpc_tree_plan* plan);
 /* launches per propagation, how many are tile runs, and their total tiles */
 int dpc_tree_plan_info(const dpc_tree_plan* plan, uint32_t* launches, uint32_t* tile_runs, uint64_t* tiles);
