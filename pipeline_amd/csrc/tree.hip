@@ -344,7 +344,7 @@ __device__ __forceinline__ float4 pick_row(const StepIn& in)
 // Computes the pieces [a, b) of one step.  Each thread has at most one slot in
 // the step: its rows go to the ring (kept passes) inside the loop, and to
 // memory (when the tile owns the slot and it changed) after the loop.
-template <unsigned MaxRing>
+template <bool HasDirty, unsigned MaxRing>
 __device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32_t a, uint32_t b, uint32_t ring_len,
                                          const float4* W, size_t ld, const uint8_t* dirty_world,
                                          const __amdgpu_buffer_rsrc_t (&rw)[4], __amdgpu_buffer_rsrc_t rdw,
@@ -378,10 +378,27 @@ __device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32
       if (dp_lds) {  // the parent was recomputed by this tile: its rows are in the ring
         dirty = true;
       } else if (valid && (!staged || in.dl)) {
-        // a kept parent that the tile left unchanged has dirty_world == 0
-        uint32_t dp = 0;
-        load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
-        dirty = (in.dl | (staged ? 0u : dp)) != 0;
+        const uint32_t pu = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(p));
+        if (!HasDirty && __builtin_amdgcn_ballot_w64(p != pu) == 0) {
+          // every slot of the wave has the same parent outside the run (the
+          // root, for a run from the top): its rows come through the scalar
+          // cache, which leaves the vector loads of later steps in flight.  The
+          // launch never writes that slot, and without dirty flags its
+          // dirty_world is not needed.
+          typedef const __attribute__((address_space(4))) float* ConstFloats;
+          const ConstFloats wc = (ConstFloats)(reinterpret_cast<uintptr_t>(W));
+          const size_t q = 4 * static_cast<size_t>(pu), r = 4 * ld;
+          p0 = make_float4(wc[q], wc[q + 1], wc[q + 2], wc[q + 3]);
+          p1 = make_float4(wc[r + q], wc[r + q + 1], wc[r + q + 2], wc[r + q + 3]);
+          p2 = make_float4(wc[2 * r + q], wc[2 * r + q + 1], wc[2 * r + q + 2], wc[2 * r + q + 3]);
+          p3 = make_float4(wc[3 * r + q], wc[3 * r + q + 1], wc[3 * r + q + 2], wc[3 * r + q + 3]);
+          dirty = true;
+        } else {
+          // a kept parent that the tile left unchanged has dirty_world == 0
+          uint32_t dp = 0;
+          load_parent(W, ld, dirty_world, p, p0, p1, p2, p3, dp);
+          dirty = (in.dl | (staged ? 0u : dp)) != 0;
+        }
       }
       quad = (h.flags & tileplan::kQuad) != 0;
       if (dirty) {
@@ -476,17 +493,17 @@ __global__ __launch_bounds__(Threads) void tree_tile_kernel(const float4* __rest
   uint32_t k = 0;
   for (; k + 2 < steps; k += 3) {
     const uint32_t pd = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, pc, C);
-    run_step<MaxRing>(A, sg, pa, pb, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
+    run_step<HasDirty, MaxRing>(A, sg, pa, pb, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
     const uint32_t pe = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, pd < n ? pd : pc, A);
-    run_step<MaxRing>(B, sg, pb, pc, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
+    run_step<HasDirty, MaxRing>(B, sg, pb, pc, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
     const uint32_t pf = load_step<HasDirty>(L, ld, parent, dirty_local, sg, n, pe < n ? pe : pc, B);
-    run_step<MaxRing>(C, sg, pc, pd, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
+    run_step<HasDirty, MaxRing>(C, sg, pc, pd, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
     pa = pd;
     pb = pe;
     pc = pf;
   }
-  if (k < steps) run_step<MaxRing>(A, sg, pa, pb, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
-  if (k + 1 < steps) run_step<MaxRing>(B, sg, pb, pc, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
+  if (k < steps) run_step<HasDirty, MaxRing>(A, sg, pa, pb, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
+  if (k + 1 < steps) run_step<HasDirty, MaxRing>(B, sg, pb, pc, ring_len, W, ld, dirty_world, rw, rdw, rows, flag);
 }
 
 // dp::math::invertTranspose (dp/math/Matmnt.h:1171-1201) of W[s]: 2x2 minors of
