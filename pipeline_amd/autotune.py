@@ -74,6 +74,17 @@ def best_cull_config(objects: int, arch: str | None = None):
     return _lib.CullConfig(entry["waves_per_cu"]) if entry else None
 
 
+def _prewarm(issue, stream, seconds=0.3):
+    """Untimed launches for `seconds` before the first candidate: the GPU's clocks
+    ramp up over tens of milliseconds, and the candidates timed first would
+    otherwise be measured slow."""
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(32):
+            issue()
+        stream.sync()
+
+
 def _time(issue, stream, reps=30, warm=5):
     for _ in range(warm):
         issue()
@@ -103,6 +114,8 @@ def tune_cull(fs: scenes.FlatScene, candidates=(0, 4, 8, 12, 16, 24, 32), copies
     _lib.check(lib.dpc_cull_get_config(C.byref(old)))
     ref = None
     rows = []
+    _prewarm(lambda: _lib.check(lib.dpc_cull(sc[0].box_a.ptr, sc[0].box_b.ptr, None, sc[0].n_objects, sc[0].planes.ptr,
+                                             sc[0].ld, vp, masks[0].ptr, stream.handle)), stream)
     for w in candidates:
         _lib.check(lib.dpc_cull_set_config(C.byref(_lib.CullConfig(w))))
         k = [0]
@@ -152,6 +165,7 @@ def tune_batch(bs: scenes.BatchedScene, candidates=None, log=print):
     plan0.bind([scene])
     plan0.cull_per_view(bs.vps, masks, stream.handle)
     ref = masks.download(stream.handle)
+    _prewarm(lambda: plan0.cull(d_vps, masks, stream.handle), stream)
     plan0.destroy()
     rows = []
     for cfg in (candidates or batch_candidates()):
