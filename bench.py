@@ -361,7 +361,7 @@ def pmc_probe():
     stream.sync()
     scene.free()
     plan.destroy()
-    fs = scenes.config1_scene()
+    fs = scenes.flat_scene_1m()
     copies = [dev.upload_scene(fs.lower, fs.upper, fs.matrices) for _ in range(4)]
     fmask = dev.DeviceArray(dev.mask_words(fs.n), np.uint64)
     vp16 = dev.vp_arg(fs.vp)
@@ -532,7 +532,7 @@ def main():
 
     # ---------------- flat single frustum (HBM roofline) ----------------
     if not args.skip_extra:
-        fs = scenes.config1_scene()
+        fs = scenes.flat_scene_1m()
         copies = [dev.upload_scene(fs.lower, fs.upper, fs.matrices) for _ in range(4)]
         fmask = [dev.DeviceArray(dev.mask_words(fs.n), np.uint64) for _ in range(4)]
         vp16 = dev.vp_arg(fs.vp)

@@ -206,7 +206,7 @@ def main(argv=None):
     table.setdefault("cull", {})
     t0 = time.time()
     for n in (1 << 20,):
-        fs = scenes.config1_scene(n=n)
+        fs = scenes.flat_scene_1m(n=n)
         best, rows = tune_cull(fs, log=log)
         table["cull"][cull_bucket(n)] = {"waves_per_cu": best["waves_per_cu"], "us": best["us"], "gbs": best["gbs"],
                                          "candidates": rows}

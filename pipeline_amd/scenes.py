@@ -165,13 +165,13 @@ def flat_scene(n: int, seed: int, center=(-100.0, 100.0), fovy=60.0, aspect=16 /
     return FlatScene(lower, upper, mats, vp, name)
 
 
-def config0_scene(seed: int = 0) -> FlatScene:
+def flat_scene_10k(seed: int = 0) -> FlatScene:
     """10,000 objects, centres U[-100,100]^3, one perspective frustum (fov 60, 16:9,
     near 0.1, far 500) at the origin looking down -z."""
     return flat_scene(10_000, seed, (-100.0, 100.0), 60.0, 16 / 9, 0.1, 500.0, name="flat-10k")
 
 
-def config1_scene(seed: int = 1, n: int = 1 << 20) -> FlatScene:
+def flat_scene_1m(seed: int = 1, n: int = 1 << 20) -> FlatScene:
     """1,048,576 objects, centres U[-1000,1000]^3, fov 45, far 2000, random eye in
     [-50,50]^3 and random look direction."""
     rng = np.random.default_rng(seed + 1_000_003)

@@ -83,7 +83,7 @@ def test_per_view_matches_oracle(orc):
     b.free()
 
 
-def test_batch_equals_per_view_config3(orc):
+def test_batch_equals_per_view_2m_objects_64_frusta(orc):
     """Full-size batched workload: 2,097,152 objects, 64 groups, 64 frusta."""
     bs = scenes.batched_scene()
     b = Batch(bs)
@@ -95,7 +95,7 @@ def test_batch_equals_per_view_config3(orc):
     assert np.array_equal(batched, per_view)
     want = oracle_pairs(orc, bs)
     n_mis = mismatches(batched, want, bs.mask_words * 64)
-    print(f"config3: {bs.n} objects, {len(bs.pair_group)} pairs, {bs.tests} tests, "
+    print(f"batched 2M x 64: {bs.n} objects, {len(bs.pair_group)} pairs, {bs.tests} tests, "
           f"{int(np.unpackbits(want.view(np.uint8)).sum())} visible, {n_mis} mismatching bits")
     assert n_mis == 0
     b.free()

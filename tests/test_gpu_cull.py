@@ -32,20 +32,20 @@ def oracle_mask(orc, lower, upper, matrices, vp, xform=None):
     return orc.cull_group(l4, e4, x, matrices, vp)
 
 
-def test_config0_flat_10k(orc):
-    s = scenes.config0_scene()
+def test_flat_10k(orc):
+    s = scenes.flat_scene_10k()
     got = gpu_cull_flat(s.lower, s.upper, s.matrices, s.vp)
     want = oracle_mask(orc, s.lower, s.upper, s.matrices, s.vp)
     assert mismatches(got, want, s.n) == 0
     assert 0 < bits_of(want, s.n).sum() < s.n
 
 
-def test_config1_flat_1m(orc, cull_shape):
-    s = scenes.config1_scene()
+def test_flat_1m(orc, cull_shape):
+    s = scenes.flat_scene_1m()
     got = gpu_cull_flat(s.lower, s.upper, s.matrices, s.vp)
     want = oracle_mask(orc, s.lower, s.upper, s.matrices, s.vp)
     n_mis = mismatches(got, want, s.n)
-    print(f"config1: {s.n} objects, {int(bits_of(want, s.n).sum())} visible, {n_mis} mismatching bits")
+    print(f"flat 1M: {s.n} objects, {int(bits_of(want, s.n).sum())} visible, {n_mis} mismatching bits")
     assert n_mis == 0
 
 

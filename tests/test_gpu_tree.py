@@ -108,7 +108,7 @@ def test_propagate_matches_oracle(orc, n_nodes, seed, kw, use_plan):
     t.free()
 
 
-def test_config2_tree_1m_and_cull(orc):
+def test_tree_1m_and_cull(orc):
     """1,000,000 nodes (depth <= 16, geometric children), then cull of ~2.4M leaf objects."""
     ts = scenes.tree_scene(1_000_000, 2)
     t = DeviceTree(ts)
@@ -122,7 +122,7 @@ def test_config2_tree_1m_and_cull(orc):
     w64 = orc.tree_compute_f64(offsets, entries, ts.local_by_index().astype(np.float64))[ts.index_of_slot]
     scale = np.abs(w64[1:]).max(axis=(1, 2), keepdims=True)
     rel = float((np.abs(got[1:].astype(np.float64) - w64[1:]) / scale).max())
-    print(f"config2: {ts.n_slots - 1} nodes, {len(ts.level_offsets) - 1} levels, max ulp vs FP32 oracle "
+    print(f"tree 1M: {ts.n_slots - 1} nodes, {len(ts.level_offsets) - 1} levels, max ulp vs FP32 oracle "
           f"{int(ulps.max())}, max relative error vs FP64 {rel:.3e}")
     assert ulps.max() == 0
     assert rel < 1e-4
@@ -137,7 +137,7 @@ def test_config2_tree_1m_and_cull(orc):
     want_mask = orc.cull_group(l4, e4, ts.obj_slot, want_by_slot, ts.vp)
     m = ts.obj_slot.shape[0]
     n_mis = mismatches(mask, want_mask, m)
-    print(f"config2 cull: {m} objects, {int(np.unpackbits(want_mask.view(np.uint8)).sum())} visible, "
+    print(f"tree 1M cull: {m} objects, {int(np.unpackbits(want_mask.view(np.uint8)).sum())} visible, "
           f"{n_mis} mismatching bits")
     assert n_mis == 0
     box.box_a.free()
@@ -195,7 +195,7 @@ def test_dirty_propagation(orc):
 
 
 @pytest.mark.parametrize("tile_slots", [1024, 512, 256])
-def test_plan_tiles_config2_tree_and_dirty_frames(orc, tile_slots):
+def test_plan_tiles_tree_1m_and_dirty_frames(orc, tile_slots):
     """The 1,000,000-node tree through the plan: its deep levels form a tile run;
     a full compute and then two partial ones (a few hundred dirty locals) give
     the oracle's world matrices and events bit for bit."""
