@@ -353,14 +353,17 @@ __device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32
   bool store = false, quad = false;
   float4 w0 = make_float4(0.f, 0.f, 0.f, 0.f), w1 = w0, w2 = w0, w3 = w0;
   // a < b <= n for every step of a valid plan; b <= n always (load_step)
-  SegHead h = seg_head(sg, a < b ? a : 0u);
+  const uint32_t a0 = a < b ? a : 0u;
+  SegHead h = seg_head(sg, a0);
+  uint2 pr = *reinterpret_cast<const uint2*>(&sg[a0].pfirst);  // the parent range: pfirst, pcount | pring << 16
   for (uint32_t j = a; j < b; ++j) {
-    // the next piece's descriptor, read while this one is computed
-    const uint4 hn = *reinterpret_cast<const uint4*>(&sg[j + 1 < b ? j + 1 : j]);
+    // the next piece's descriptor (both halves), read while this one is computed
+    const uint32_t jn = j + 1 < b ? j + 1 : j;
+    const uint4 hn = *reinterpret_cast<const uint4*>(&sg[jn]);
+    const uint2 prn = *reinterpret_cast<const uint2*>(&sg[jn].pfirst);
     if (in.seg == j) {
-      const uint2 pr = *reinterpret_cast<const uint2*>(&sg[j].pfirst);
       const uint32_t pfirst = __builtin_amdgcn_readfirstlane(pr.x);
-      const uint32_t pp = __builtin_amdgcn_readfirstlane(pr.y);  // pcount | pring << 16
+      const uint32_t pp = __builtin_amdgcn_readfirstlane(pr.y);
       const uint32_t pcount = pp & 0xFFFFu, pring = pp >> 16;
       const uint64_t p = in.p;
       const bool valid = p < ld;  // a parent outside the arrays is never read (as in propagate_slot)
@@ -432,6 +435,7 @@ __device__ __forceinline__ void run_step(const StepIn& in, const Seg* sg, uint32
     // operations execute in order
     if (h.flags & tileplan::kBarrier) lds_barrier();
     h = decode_head(hn);
+    pr = prn;
   }
   // one row plane per thread of a kQuad slot, all four otherwise
   const uint32_t off = store ? in.slot * 16u : kNoStore;
