@@ -5,15 +5,14 @@
 // A run of consecutive passes in which every parent of a pass lies in the
 // previous pass ("linked" passes) is one launch of tree_tile_kernel:
 //
-//   - the run's last pass is cut into contiguous slot ranges, one per tile
-//     (workgroup); a tile's range in every earlier pass of the run is the
-//     contiguous range spanned by the ancestors of its last-pass range, from
-//     the ancestor of its first slot to the ancestor of the next tile's first
-//     slot inclusive (the first tile starts at the beginning of each pass and
-//     the last one ends at its end, so every slot is in some tile);
-//   - neighbouring tiles therefore share at most one slot per pass; both
-//     compute it (the same inputs give the same result) and only the tile in
-//     whose half-open range [start, next tile's start) it lies writes it;
+//   - the run's forest is cut into tiles (workgroups) that are consecutive
+//     intervals of its depth-first post-order; an interval is a contiguous
+//     slot range of every pass of the run (build_run), so every slot is in
+//     exactly one tile's owned range;
+//   - a parent outside a tile's interval is the first slot of its pass after
+//     the interval: the tile recomputes it as one extra slot (the same inputs
+//     give the same result) and only the owning tile writes it, so tiles
+//     share at most one slot per pass;
 //   - a tile's slots, pass after pass, are cut into steps of `threads` slots,
 //     one slot per thread; a step holds pieces ("segments") of one or more
 //     passes, so small passes share a step and a step's loads are issued
