@@ -336,7 +336,8 @@ def measured_traffic(kernel: str):
 def pmc_probe():
     """Kernels for the counter passes: a plain device copy of PMC_COPY_BYTES
     (calibrates FETCH_SIZE / WRITE_SIZE to bytes), then the batched launch of
-    the headline workload and the flat single-frustum launch, 5 times each."""
+    the headline workload, the flat single-frustum launch and the propagation
+    of the 1,000,000-node tree, 5 times each."""
     from pipeline_amd import _lib, scenes
     from pipeline_amd import device as dev
     from pipeline_amd import autotune
@@ -370,8 +371,29 @@ def pmc_probe():
         _lib.check(lib.dpc_cull(s.box_a.ptr, s.box_b.ptr, None, s.n_objects, s.planes.ptr, s.ld, vp16, fmask.ptr,
                                 stream.handle))
     stream.sync()
+    for c in copies:
+        c.free()
+    fmask.free()
+    ts = scenes.tree_scene(1_000_000, 2)
+    d_local = dev.DeviceArray.from_host(dev.matrices_to_planes(ts.local))
+    w0 = np.zeros((4, ts.n_slots, 4), np.float32)
+    w0[:, 0, :] = np.eye(4, dtype=np.float32)
+    d_world = dev.DeviceArray.from_host(w0)
+    d_dirty = dev.DeviceArray(((ts.n_slots + 3) // 4) * 4, np.uint8)
+    d_dirty.fill_u32(0, stream.handle)
+    tplan = C.c_void_p()
+    par_h = np.ascontiguousarray(ts.parent_slot, np.uint32)
+    off_h = np.ascontiguousarray(ts.level_offsets, np.uint64)
+    _lib.check(lib.dpc_tree_plan_create(par_h.ctypes.data, ts.n_slots, off_h.ctypes.data, len(off_h) - 1, 0,
+                                        C.byref(tplan)))
+    for _ in range(5):
+        _lib.check(lib.dpc_tree_propagate_plan(tplan, d_local.ptr, d_world.ptr, None, d_dirty.ptr, stream.handle))
+    stream.sync()
+    lib.dpc_tree_plan_destroy(tplan)
+    for d in (d_local, d_world, d_dirty):
+        d.free()
     stream.destroy()
-    log("pmc probe: 3 copies of 512 MiB, 5 batched launches, 5 flat launches")
+    log("pmc probe: 3 copies of 512 MiB, 5 batched launches, 5 flat launches, 5 tree propagations")
 
 
 def main():
