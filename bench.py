@@ -54,8 +54,8 @@ def parse():
     ap.add_argument("--skip-extra", action="store_true", help="only the batched and per-view lines")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="launch only a 512 MiB device copy, the batched kernel and the flat kernel a few times "
-                         "each (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes); prints no JSON")
+                    help="launch only a 512 MiB device copy, the batched kernel, the flat kernel and the tree "
+                         "propagation a few times each (for rocprofv3 --pmc passes); prints no JSON")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
     ap.add_argument("--serve-streams", type=int, default=8, help="streams of the serving measurement")
     ap.add_argument("--serve-depth", type=int, default=2, help="batches in flight per stream in the serving measurement")

@@ -181,6 +181,29 @@ struct RawObj {
   float4 r0, r1, r2, r3;
 };
 
+// Loads through a pointer known to address global memory.  Pointers read from
+// a table in memory (a scene descriptor) are generic to the compiler, and a
+// generic (flat) load makes it wait for every load in flight before using any
+// of them, which would also wait for the prefetch of the next tile.
+__device__ __forceinline__ float4 gload(const float4* p)
+{
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  const v4 v = *(const __attribute__((address_space(1))) v4*)(reinterpret_cast<uintptr_t>(p));
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ float2 gload(const float2* p)
+{
+  typedef float v2 __attribute__((ext_vector_type(2)));
+  const v2 v = *(const __attribute__((address_space(1))) v2*)(reinterpret_cast<uintptr_t>(p));
+  return make_float2(v.x, v.y);
+}
+
+__device__ __forceinline__ uint32_t gload(const uint32_t* p)
+{
+  return *(const __attribute__((address_space(1))) uint32_t*)(reinterpret_cast<uintptr_t>(p));
+}
+
 // Loads object i with matrix m.  The loads are unconditional so that no branch
 // separates them from the compute (which lets the compiler wait for exactly
 // the loads a tile needs): the caller passes indices that are in range, and
@@ -190,12 +213,12 @@ __device__ __forceinline__ RawObj load_raw(const float4* __restrict__ box_a, con
                                            const float4* __restrict__ planes, size_t ld, size_t i, size_t m)
 {
   RawObj r;
-  r.a = box_a[i];
-  r.b = box_b[i];
-  r.r0 = planes[m];
-  r.r1 = planes[ld + m];
-  r.r2 = planes[2 * ld + m];
-  r.r3 = planes[3 * ld + m];
+  r.a = gload(box_a + i);
+  r.b = gload(box_b + i);
+  r.r0 = gload(planes + m);
+  r.r1 = gload(planes + ld + m);
+  r.r2 = gload(planes + 2 * ld + m);
+  r.r3 = gload(planes + 3 * ld + m);
   return r;
 }
 
@@ -254,7 +277,7 @@ __device__ __forceinline__ Slot make_slot(size_t i, size_t n, const uint32_t* __
   Slot s;
   s.valid = i < n;
   s.i = s.valid ? i : 0;
-  const size_t m = xform ? static_cast<size_t>(xform[s.i]) : s.i;
+  const size_t m = xform ? static_cast<size_t>(gload(xform + s.i)) : s.i;
   s.bad_matrix = m >= ld;
   s.m = s.bad_matrix ? 0 : m;
   return s;

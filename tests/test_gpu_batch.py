@@ -101,6 +101,46 @@ def test_batch_equals_per_view_2m_objects_64_frusta(orc):
     b.free()
 
 
+def test_degenerate_matrices_and_frusta(orc):
+    """Inputs off the common path: waves with one projective object matrix, a
+    whole group of them, last columns of -0, M33 = 2, zero extents, and frusta
+    whose row 3 holds inf, NaN, -inf or a huge finite value (some in the middle
+    of an item's frusta).  Batched launches of several shapes and the per-view
+    launch all equal the oracle bit for bit."""
+    bs = small_scene(13)
+    rng = np.random.default_rng(5)
+    n = bs.n
+    m = bs.matrices
+    proj = rng.choice(n, n // 500, replace=False)
+    m[proj, 0:3, 3] = rng.uniform(-0.01, 0.01, (proj.size, 3)).astype(F)
+    negz = rng.choice(n, n // 50, replace=False)
+    m[negz, 0:3, 3] = F(-0.0)
+    w2 = rng.choice(n, n // 1000, replace=False)
+    m[w2, 3, 3] = F(2.0)
+    flat = rng.choice(n, n // 100, replace=False)
+    bs.upper[flat] = bs.lower[flat]
+    # one group entirely projective: all of its waves take the full test
+    g = int(np.argmax(bs.group_count))
+    a, c = int(bs.group_first[g]), int(bs.group_count[g])
+    m[a:a + c, 2, 3] = F(1e-3)
+    nf = bs.vps.shape[0]
+    bs.vps[1 % nf, 3, 1] = np.inf
+    bs.vps[3 % nf, 3, 2] = np.nan
+    bs.vps[5 % nf, 3, 0] = F(3e38)
+    bs.vps[7 % nf, 3, 3] = -np.inf
+    want = oracle_pairs(orc, bs)
+    for cfg in [None, (1, 256, 8, 0), (2, 128, 16, 4), (4, 64, 8, 3)]:
+        b = Batch(bs, None if cfg is None else _lib.BatchConfig(*cfg))
+        b.masks.fill_u32(0xDEADBEEF)
+        b.run_batch()
+        assert mismatches(b.result(), want, bs.mask_words * 64) == 0, f"batched {cfg}"
+        if cfg is None:
+            b.masks.fill_u32(0xDEADBEEF)
+            b.run_per_view()
+            assert mismatches(b.result(), want, bs.mask_words * 64) == 0, "per view"
+        b.free()
+
+
 def test_graph_replay_with_moving_frusta(orc):
     """The batched launch captured once in a hipGraph and replayed while the
     frusta change in device memory between replays (a camera moving per frame):
