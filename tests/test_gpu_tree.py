@@ -90,6 +90,13 @@ def oracle_world(orc, ts):
     # two levels above the single-workgroup size, then a small one whose parents
     # were written by an earlier launch
     (6_500, 9, {"roots": 3_000, "p": 0.99}),
+    # deeper than one tile run (several runs, each first pass reading parents
+    # written by the previous launch), a flat two-level tree of 100k + 100k
+    # nodes, and a tree that grows from a single top-level transform
+    (4_000, 11, {"max_depth": 120, "p": 0.97, "max_children": 2, "roots": 30}),
+    (6_000, 14, {"max_depth": 220, "p": 0.995, "max_children": 2, "roots": 16}),
+    (200_000, 12, {"max_depth": 2, "roots": 100_000, "p": 0.9}),
+    (60_000, 13, {"max_depth": 40, "p": 0.5, "max_children": 8, "roots": 1}),
 ])
 @pytest.mark.parametrize("use_plan", [None, 1024, 512, 256], ids=["levels", "plan1024", "plan512", "plan256"])
 def test_propagate_matches_oracle(orc, n_nodes, seed, kw, use_plan):
