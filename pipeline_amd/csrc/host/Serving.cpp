@@ -5,17 +5,15 @@
 // (scene, frusta, matrix delta) from any thread; a scheduler thread forms
 // batches continuously.  Each stream holds up to batches_per_stream batches in
 // flight, each with an arena of its own.  Whenever an arena is free, the
-// scheduler takes for its stream, in arrival order, up to kMaxRounds requests
-// per scene, until the arena is full.  The k-th request of each scene in a
-// batch belongs to round k.  A round scatters its requests' deltas into the
-// scenes' matrices and then tests each of its scenes against its request's
-// frusta in one launch (dpc::launch_serve); rounds run in stream order.  A
-// scene with requests in flight is bound to the stream that runs them and
-// joins only batches of that stream.  Stream order then keeps each scene's
-// requests in submission order (each sees the deltas of the earlier ones)
-// without waiting for them to finish.  A batch ends by copying each request's
-// masks to its caller's buffer.  Batches on different streams overlap on the
-// device.
+// scheduler takes for its stream, in arrival order, at most one request per
+// scene, until the arena is full.  A scene with requests in flight is bound to
+// the stream that runs them and joins only batches of that stream: stream order
+// then keeps each scene's requests in submission order (each sees the deltas
+// of the earlier ones) without waiting for them to finish.  A batch copies the
+// deltas to the device and scatters them into the scenes' matrices, tests every
+// scene of the batch against its request's frusta in one launch
+// (dpc::launch_serve), then copies each request's masks to its caller's
+// buffer.  Batches on different streams overlap on the device.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -42,7 +40,6 @@ namespace {
 constexpr uint32_t kMaxScenes = 64;
 constexpr uint32_t kMaxBatch = 64;
 constexpr uint32_t kMaxFrusta = 64;
-constexpr uint32_t kMaxRounds = 4;  // requests of one scene per batch
 // the pinned staging area of one batch's descriptors (entries, pairs, frusta)
 constexpr uint64_t kHostArena = 4096 + kMaxBatch * 32 + kMaxBatch * kMaxFrusta * (16 + 64) + 3 * 256;
 
@@ -164,47 +161,25 @@ int hip_fail(hipError_t e, const char* what)
 // calls run with it released (only the scheduler thread touches slots).
 bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lock)
 {
-  // Requests in arrival order; a scene may contribute up to kMaxRounds of them.
-  // The k-th request of each scene goes to round k: a round scatters its
-  // requests' deltas and then culls them in one launch, and rounds run in
-  // stream order, so each request sees exactly the deltas of its scene's
-  // earlier requests.  Once a request of a scene is passed over, the scene's
-  // later requests wait too.
   std::vector<std::shared_ptr<Request>> pick;
-  std::vector<uint32_t> pick_round;
-  std::vector<uint32_t> scene_count(scenes.size(), 0);
-  std::vector<bool> scene_blocked(scenes.size(), false);
+  std::vector<bool> scene_taken(scenes.size(), false);
   uint64_t bytes = 0;
   for (auto it = pending.begin(); it != pending.end() && pick.size() < cfg.max_batch_requests;) {
     const auto& r = *it;
     Scene& sc = scenes[r->scene];
     const bool other_stream = sc.inflight != 0 && sc.stream != slot.stream_index;
-    if (!sc.ready || other_stream || scene_blocked[r->scene] || scene_count[r->scene] == kMaxRounds) {
-      scene_blocked[r->scene] = true;  // later requests of this scene wait behind this one
+    if (!sc.ready || other_stream || scene_taken[r->scene]) {
+      scene_taken[r->scene] = true;  // later requests of this scene wait behind this one
       ++it;
       continue;
     }
     if (bytes + r->dev_bytes > slot.cap) break;  // admission: the arena (memory budget) is full
+    scene_taken[r->scene] = true;
     bytes += r->dev_bytes;
-    pick_round.push_back(scene_count[r->scene]++);
     pick.push_back(r);
     it = pending.erase(it);
   }
   if (pick.empty()) return false;
-  // round by round (a stable order keeps arrival order within a round)
-  {
-    std::vector<size_t> order(pick.size());
-    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return pick_round[a] < pick_round[b]; });
-    std::vector<std::shared_ptr<Request>> sorted(pick.size());
-    std::vector<uint32_t> sorted_round(pick.size());
-    for (size_t k = 0; k < order.size(); ++k) {
-      sorted[k] = pick[order[k]];
-      sorted_round[k] = pick_round[order[k]];
-    }
-    pick.swap(sorted);
-    pick_round.swap(sorted_round);
-  }
   const uint32_t batch = ++batch_seq;
   const double t_admit = now();
   for (auto& r : pick) {
@@ -251,20 +226,11 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   auto* pairs = reinterpret_cast<dpc::KPair*>(slot.h_arena + pairs_off);
   auto* vps = reinterpret_cast<float*>(slot.h_arena + vps_off);
   std::vector<uint64_t> mask_off(pick.size());  // byte offset of each request's masks in the device arena
-  // round r holds entries [round_begin[r], round_begin[r + 1]), its cost prefix
-  // starting at 0 (each round is one launch)
-  std::vector<size_t> round_begin;
-  std::vector<uint64_t> round_cost;
   uint64_t cost = 0, mask_word = 0, tests = 0;
   uint32_t p = 0;
   for (size_t k = 0; k < pick.size(); ++k) {
     const Request& r = *pick[k];
     const Scene& sc = snapshot[k];
-    if (k == 0 || pick_round[k] != pick_round[k - 1]) {
-      if (k) round_cost.push_back(cost);
-      round_begin.push_back(k);
-      cost = 0;
-    }
     dpc::ServeEntry& en = entries[k];
     en.scene = r.scene;
     en.pair_begin = p;
@@ -283,59 +249,46 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
     }
     tests += sc.n * r.n_frusta;
   }
-  round_cost.push_back(cost);
-  round_begin.push_back(pick.size());
   const uint64_t end = masks_off + 8 * mask_word;
   int rc = DPC_OK;
   if (end > slot.cap) rc = fail(DPC_ERR_OVER_BUDGET, "batch larger than its arena");
-  // issue-time parts: 0 delta copies, 1 delta scatter launches, 2 descriptors,
-  // 3 cull launches, 4 mask copies and the event
   double t_part[6];
-  double copy_us = 0.0, scatter_us = 0.0, launch_us = 0.0;
+  double copy_us = 0.0;
   t_part[0] = now();
+  // deltas: copy to the device and scatter into the scenes' matrices
+  for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
+    const Request& r = *pick[k];
+    if (!r.n_delta) continue;
+    if (r.didx_dev) {  // pinned: the scatter reads the caller's arrays directly
+      rc = dpc::launch_scatter_host(snapshot[k].planes, snapshot[k].n_mat, r.didx_dev, r.dmats_dev, r.n_delta, s);
+      continue;
+    }
+    const double tc = now();
+    e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(slot.d_arena + mat_off[k], r.dmats, 64ull * r.n_delta, hipMemcpyHostToDevice, s);
+    copy_us += now() - tc;
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "delta upload");
+      break;
+    }
+    rc = dpc_matrices_scatter(snapshot[k].planes, snapshot[k].n_mat,
+                              reinterpret_cast<const uint32_t*>(slot.d_arena + idx_off[k]),
+                              reinterpret_cast<const float*>(slot.d_arena + mat_off[k]), r.n_delta, s);
+  }
+  t_part[2] = now();
+  t_part[1] = t_part[0] + copy_us;  // parts 0 and 1: copies, then the scatter launches (interleaved in time)
   if (rc == DPC_OK) {
     e = hipMemcpyAsync(slot.d_arena, slot.h_arena, desc_end, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(e, "descriptor upload");
   }
-  const double t_desc = now() - t_part[0];
-  for (size_t q = 0; q + 1 < round_begin.size() && rc == DPC_OK; ++q) {
-    // the round's deltas: copy to the device and scatter into the scenes' matrices
-    for (size_t k = round_begin[q]; k < round_begin[q + 1] && rc == DPC_OK; ++k) {
-      const Request& r = *pick[k];
-      if (!r.n_delta) continue;
-      const double tc = now();
-      if (r.didx_dev) {  // pinned: the scatter reads the caller's arrays directly
-        rc = dpc::launch_scatter_host(snapshot[k].planes, snapshot[k].n_mat, r.didx_dev, r.dmats_dev, r.n_delta, s);
-        scatter_us += now() - tc;
-        continue;
-      }
-      e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
-      if (e == hipSuccess)
-        e = hipMemcpyAsync(slot.d_arena + mat_off[k], r.dmats, 64ull * r.n_delta, hipMemcpyHostToDevice, s);
-      const double ts = now();
-      copy_us += ts - tc;
-      if (e != hipSuccess) {
-        rc = hip_fail(e, "delta upload");
-        break;
-      }
-      rc = dpc_matrices_scatter(snapshot[k].planes, snapshot[k].n_mat,
-                                reinterpret_cast<const uint32_t*>(slot.d_arena + idx_off[k]),
-                                reinterpret_cast<const float*>(slot.d_arena + mat_off[k]), r.n_delta, s);
-      scatter_us += now() - ts;
-    }
-    if (rc != DPC_OK) break;
-    // then every request of the round in one launch
-    const double tl = now();
-    rc = dpc::launch_serve(slot.d_arena + entries_off + sizeof(dpc::ServeEntry) * round_begin[q],
-                           static_cast<uint32_t>(round_begin[q + 1] - round_begin[q]), round_cost[q],
+  t_part[3] = now();
+  if (rc == DPC_OK) {
+    rc = dpc::launch_serve(slot.d_arena + entries_off, static_cast<uint32_t>(pick.size()), cost,
                            slot.d_arena + pairs_off, d_scene_table, reinterpret_cast<const float*>(slot.d_arena + vps_off),
                            reinterpret_cast<uint64_t*>(slot.d_arena + masks_off), K, cfg.waves_per_cu, s);
-    launch_us += now() - tl;
   }
-  t_part[1] = t_part[0] + copy_us;
-  t_part[2] = t_part[1] + scatter_us;
-  t_part[3] = t_part[2] + t_desc;
-  t_part[4] = t_part[3] + launch_us;
+  t_part[4] = now();
   // each request's masks go straight to the caller's buffer
   for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
     const Request& r = *pick[k];
@@ -347,7 +300,7 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
     if (e != hipSuccess) rc = hip_fail(e, "completion event");
   }
   const double t_issue1 = now();
-  t_part[5] = t_part[4] + (t_issue1 - t_part[0] - copy_us - scatter_us - t_desc - launch_us);
+  t_part[5] = t_issue1;
   lock.lock();
   stats.issue_us += t_issue1 - t_issue0;
   stats.issue_max_us = std::max(stats.issue_max_us, t_issue1 - t_issue0);

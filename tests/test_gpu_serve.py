@@ -74,11 +74,6 @@ def test_serve_mix_equals_alone_and_oracle(orc):
     assert max(info.batch_requests for info, _ in mix) > 1
     spans = sorted({(info.batch, info.admit_us, info.done_us) for info, _ in mix})
     assert any(b[1] < a[2] and a[0] != b[0] for a, b in zip(spans, spans[1:])), "no two batches were in flight together"
-    # a batch carried several requests of one scene (one per round)
-    by_batch = {}
-    for r, (info, _) in zip(reqs, mix):
-        by_batch.setdefault(info.batch, []).append(r.scene)
-    assert any(len(v) != len(set(v)) for v in by_batch.values()), "no batch held two requests of a scene"
     # a scene's next request is admitted (on the same stream) before its previous one is done
     by_scene = {}
     for r, (info, _) in zip(reqs, mix):
@@ -102,27 +97,6 @@ def test_serve_mix_equals_alone_and_oracle(orc):
         alone = _run_alone(sc, [reqs[i] for i in mine])
         for i, m in zip(mine, alone):
             assert np.array_equal(m, mix[i][1]), f"scene {s} request {i}: alone != mix"
-
-
-def test_serve_rounds_of_one_scene(orc):
-    """Requests of one scene queued together run as rounds of one batch (at
-    most four per batch): each round scatters its delta and then culls, so
-    every request sees exactly the deltas submitted before it."""
-    scenes = S.serving_scenes(seed=21, n_scenes=1, size_range=(20_000, 20_000))
-    reqs = S.serving_requests(scenes, seed=22, n_requests=14, rate_per_s=0.0)
-    mix, stats = _run_mix(scenes, reqs, streams=1, max_batch=8)
-    assert all(info.status == 0 for info, _ in mix)
-    assert max(info.batch_requests for info, _ in mix) > 1
-    assert max(info.batch_requests for info, _ in mix) <= 4
-    rep = Replay(orc, scenes)
-    n = scenes[0].n
-    words = (n + 63) // 64
-    bad = 0
-    for r, (info, m) in zip(reqs, mix):
-        want = rep.apply_and_cull(r)
-        for f in range(r.vps.shape[0]):
-            bad += mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
-    assert bad == 0, f"{bad} visibility bits differ from the oracle"
 
 
 def test_serve_pinned_deltas_read_directly(orc):
