@@ -112,6 +112,91 @@ def run_timed(dev, lib, stream, steps, warmup, issue):
     return wall, dev_ms, dev_ms
 
 
+def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
+    """The steps beside the path, on the tree workload's data: the group bounding
+    box of its 2.37M objects, the result diff of its cull mask, the inverse
+    transpose of its 1M world matrices, and propagation with 1 % of the local
+    matrices dirty.  Kernel time per call from HIP events over 50 calls; their
+    parity with the oracle is what tests/test_gpu_reduce.py, test_gpu_invert.py
+    and test_gpu_tree.py check."""
+    from pipeline_amd import _lib
+
+    out = {}
+    n_slots = ts.n_slots
+    m = objs.n_objects
+    # group bounding box (ManagerBitSet::calculateBoundingBox)
+    ws = dev.DeviceArray(max(lib.dpc_bbox_workspace(m), 4), np.uint8)
+    box6 = dev.DeviceArray(8, np.float32)
+
+    def issue_bbox():
+        _lib.check(lib.dpc_bbox(objs.box_a.ptr, objs.box_b.ptr, objs.xform_addr, m, d_world.ptr, n_slots, box6.ptr,
+                                ws.ptr, stream.handle))
+
+    _, k_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_bbox)
+    leaves = len(np.unique(ts.obj_slot))
+    b = m * (24 + 4) + 64 * leaves
+    out["bbox"] = dict(objects=m, us=k_ms * 1e3, bytes=b, gbs=b / (k_ms * 1e-3) / 1e9)
+    ws.free()
+    box6.free()
+    # result diff (ResultBitSet::updateChanged): the cull mask against a copy with 1 % of its bits flipped
+    words = dev.mask_words(m)
+    new_h = tmask.download(stream.handle)
+    rng = np.random.default_rng(3)
+    flip = np.unique(rng.integers(0, m, m // 100))
+    old_h = new_h.copy()
+    np.bitwise_xor.at(old_h, flip >> 6, np.left_shift(np.uint64(1), (flip & 63).astype(np.uint64)))
+    d_old = dev.DeviceArray.from_host(old_h)
+    d_idx = dev.DeviceArray(max(flip.size, 1), np.uint32)
+    d_cnt = dev.DeviceArray(1, np.uint32)
+    ws = dev.DeviceArray(max(lib.dpc_result_diff_workspace(m), 4), np.uint8)
+
+    def issue_diff():
+        _lib.check(lib.dpc_result_diff(d_old.ptr, tmask.ptr, m, d_idx.ptr, d_cnt.ptr, 0, ws.ptr, stream.handle))
+
+    _, k_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_diff)
+    count = int(d_cnt.download(stream.handle)[0])
+    b = 16 * words + 4 * flip.size
+    out["result_diff"] = dict(bits=m, changed=count, changed_expected=int(flip.size), us=k_ms * 1e3, bytes=b,
+                              gbs=b / (k_ms * 1e-3) / 1e9)
+    for d in (d_old, d_idx, d_cnt, ws):
+        d.free()
+    # inverse transpose of every world matrix (dp::math::invertTranspose)
+    d_inv = dev.DeviceArray(4 * n_slots * 4, np.float32)
+    d_ok = dev.DeviceArray(((n_slots + 3) // 4) * 4, np.uint8)
+
+    def issue_inv():
+        _lib.check(lib.dpc_inverse_transpose(d_world.ptr, n_slots, None, n_slots, d_inv.ptr, n_slots, d_ok.ptr,
+                                             stream.handle))
+
+    _, k_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_inv)
+    b = n_slots * (64 + 64 + 1)
+    out["inverse_transpose"] = dict(matrices=n_slots, us=k_ms * 1e3, bytes=b, gbs=b / (k_ms * 1e-3) / 1e9)
+    d_inv.free()
+    d_ok.free()
+    # propagation with 1 % of the local matrices dirty (the event bytes are recomputed for every slot)
+    dl = np.zeros(((n_slots + 3) // 4) * 4, np.uint8)
+    dl[rng.choice(np.arange(1, n_slots), n_slots // 100, replace=False)] = 1
+    d_dl = dev.DeviceArray.from_host(dl)
+    d_dw = dev.DeviceArray(((n_slots + 3) // 4) * 4, np.uint8)
+    d_dw.fill_u32(0, stream.handle)
+
+    def issue_dirty():
+        _lib.check(lib.dpc_tree_propagate_plan(tplan, d_local.ptr, d_world.ptr, d_dl.ptr, d_dw.ptr, stream.handle))
+
+    _, k_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_dirty)
+    changed = int(d_dw.download(stream.handle)[1:n_slots].astype(np.int64).sum())
+    out["dirty_propagation"] = dict(nodes=n_slots - 1, dirty_local=int(dl.sum()), changed_world=changed,
+                                    us=k_ms * 1e3)
+    d_dl.free()
+    d_dw.free()
+    log("next rows: bbox {:.1f} us ({:.0f} GB/s), result diff {:.1f} us ({} of {} changed), inverse transpose "
+        "{:.1f} us ({:.0f} GB/s), propagation with 1 % dirty {:.1f} us ({} world matrices changed)".format(
+            out["bbox"]["us"], out["bbox"]["gbs"], out["result_diff"]["us"], out["result_diff"]["changed"],
+            out["result_diff"]["bits"], out["inverse_transpose"]["us"], out["inverse_transpose"]["gbs"],
+            out["dirty_propagation"]["us"], out["dirty_propagation"]["changed_world"]))
+    return out
+
+
 def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     """Serving loop: 16 resident scenes of log-uniform size in [10k, 2M] objects,
     Poisson arrivals at each rate for --serve-seconds, every request = 1-32
@@ -655,6 +740,7 @@ def main():
                               propagation_launches=t_launches.value, tile_runs=t_runs.value, tiles=t_tiles.value)
         log(f"tree: {result['tree']['us_per_step']:.1f} us/step (propagate + cull; "
             f"{result['tree']['graph_replay_us_per_step']:.1f} us as graph replays)")
+        result["next_rows"] = bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask)
         for d in (d_local, d_world, d_dirty, tmask):
             d.free()
         objs.free()

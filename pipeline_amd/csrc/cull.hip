@@ -201,14 +201,6 @@ __device__ __forceinline__ void box_take(Box6& acc, const float4& p)
   if (acc.hi[2] < p.z) acc.hi[2] = p.z;
 }
 
-__device__ __forceinline__ void box_combine(Box6& left, const Box6& right)
-{
-  for (int c = 0; c < 3; ++c) {
-    if (left.lo[c] > right.lo[c]) left.lo[c] = right.lo[c];
-    if (left.hi[c] < right.hi[c]) left.hi[c] = right.hi[c];
-  }
-}
-
 __device__ __forceinline__ Box6 box_empty()
 {
   Box6 b;
@@ -219,25 +211,85 @@ __device__ __forceinline__ Box6 box_empty()
   return b;
 }
 
-// Ordered wave reduction: lane l combines with lane l+off, whose range follows.
-__device__ __forceinline__ Box6 wave_box_reduce(Box6 b)
+constexpr unsigned kBoxThreads = 256;
+constexpr unsigned kBoxWavesPerBlock = kBoxThreads / kWave;
+constexpr unsigned kBoxWavesPerCu = 16;
+
+// A running extremum with the index of the object that set it.  Folding
+// objects in index order with strict comparisons keeps the first of equal
+// values (the reference's sequential Box4f::update, Boxnt.h:245-258); across
+// lanes, whose objects interleave, equal values are decided by the lower index.
+// Only the sign of a zero extremum depends on this (+0 and -0 compare equal).
+struct Box6i {
+  float lo[3], hi[3];
+  uint32_t ilo[3], ihi[3];
+};
+
+__device__ __forceinline__ Box6i box6i_empty()
 {
-  for (int off = 32; off >= 1; off >>= 1) {
-    Box6 r;
-    for (int c = 0; c < 3; ++c) {
-      r.lo[c] = __shfl_down(b.lo[c], off, kWave);
-      r.hi[c] = __shfl_down(b.hi[c], off, kWave);
-    }
-    if ((threadIdx.x & (kWave - 1)) + off < kWave) box_combine(b, r);
+  Box6i b;
+  for (int c = 0; c < 3; ++c) {
+    b.lo[c] = FLT_MAX;
+    b.hi[c] = -FLT_MAX;
+    b.ilo[c] = b.ihi[c] = 0xFFFFFFFFu;
   }
   return b;
 }
 
-constexpr unsigned kBoxThreads = 256;
-constexpr unsigned kBoxWavesPerBlock = kBoxThreads / kWave;
+// The smaller value, or the lower index on equal values, of (v, i) and (w, j)
+// (selects, no branches: the reductions below are latency-bound).
+__device__ __forceinline__ void take_lower(float& v, uint32_t& i, float w, uint32_t j)
+{
+  const bool take = (v > w) | ((v == w) & (j < i));
+  v = take ? w : v;
+  i = take ? j : i;
+}
 
-// Each wave scans one contiguous chunk of objects in 64-object rounds and writes
-// one partial box; partials are stored in wave order.
+__device__ __forceinline__ void take_higher(float& v, uint32_t& i, float w, uint32_t j)
+{
+  const bool take = (v < w) | ((v == w) & (j < i));
+  v = take ? w : v;
+  i = take ? j : i;
+}
+
+// Wave reduction of per-lane boxes (lane 0 receives the result).  A lane whose
+// partner lies beyond the wave reads its own values back, which changes nothing.
+__device__ __forceinline__ Box6i wave_box6i_reduce(Box6i b)
+{
+  const unsigned lane = threadIdx.x & (kWave - 1);
+  for (unsigned off = 32; off >= 1; off >>= 1) {
+    const int src = static_cast<int>(lane + off < kWave ? lane + off : lane) * 4;
+    float lo[3], hi[3];
+    uint32_t ilo[3], ihi[3];
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(b.lo[c])));
+      hi[c] = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(b.hi[c])));
+      ilo[c] = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(b.ilo[c])));
+      ihi[c] = static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src, static_cast<int>(b.ihi[c])));
+    }
+    for (int c = 0; c < 3; ++c) {
+      take_lower(b.lo[c], b.ilo[c], lo[c], ilo[c]);
+      take_higher(b.hi[c], b.ihi[c], hi[c], ihi[c]);
+    }
+  }
+  return b;
+}
+
+__device__ __forceinline__ Box6 box6_of(const Box6i& b)
+{
+  Box6 r;
+  for (int c = 0; c < 3; ++c) {
+    r.lo[c] = b.lo[c];
+    r.hi[c] = b.hi[c];
+  }
+  return r;
+}
+
+// Each wave scans one contiguous chunk of objects, lane l taking objects
+// begin + l, begin + l + 64, ... (coalesced loads).  A lane first takes the box
+// of its object's eight corners, then folds it into its running box with the
+// object's index; one wave reduction per chunk gives the chunk's box.  Partials
+// are stored in wave order, i.e. in index order of their chunks.
 __global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4* __restrict__ box_a,
                                                                    const float2* __restrict__ box_b,
                                                                    const uint32_t* __restrict__ xform, size_t n,
@@ -249,51 +301,91 @@ __global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4*
   const size_t begin = wave * chunk;
   size_t end = begin + chunk;
   if (end > n) end = n;
-  Box6 acc = box_empty();
-  for (size_t base = begin; base < end; base += kWave) {
-    const size_t i = base + lane;
+  Box6i acc = box6i_empty();
+  for (size_t i = begin + lane; i < end; i += kWave) {
+    const size_t m = xform ? static_cast<size_t>(xform[i]) : i;
+    if (m >= ld) continue;  // (an object whose matrix index is outside the array adds nothing)
+    const float4 r0 = planes[m], r1 = planes[ld + m], r2 = planes[2 * ld + m], r3 = planes[3 * ld + m];
+    // ManagerBitSet.cpp:282-294: v0 = lower * M, x/y/z = extent[k] * row k, same add tree as the cull.
+    const Obb o = make_obb(box_a[i], box_b[i], r0, r1, r2, r3);
+    const float4 v1 = add4(o.p, o.ex);
+    const float4 v2 = add4(o.p, o.ey);
+    const float4 v3 = add4(v1, o.ey);
+    const float4 v4 = add4(o.p, o.ez);
+    const float4 v5 = add4(v1, o.ez);
+    const float4 v6 = add4(v2, o.ez);
+    const float4 v7 = add4(v3, o.ez);
     Box6 mine = box_empty();
-    if (i < end) {
-      const size_t m = xform ? static_cast<size_t>(xform[i]) : i;
-      if (m < ld) {
-        const float4 r0 = planes[m], r1 = planes[ld + m], r2 = planes[2 * ld + m], r3 = planes[3 * ld + m];
-        const float4 a = box_a[i];
-        const float2 b = box_b[i];
-        // ManagerBitSet.cpp:282-294: v0 = lower * M, x/y/z = extent[k] * row k, same add tree as the cull.
-        const Obb o = make_obb(a, b, r0, r1, r2, r3);
-        const float4 v1 = add4(o.p, o.ex);
-        const float4 v2 = add4(o.p, o.ey);
-        const float4 v3 = add4(v1, o.ey);
-        const float4 v4 = add4(o.p, o.ez);
-        const float4 v5 = add4(v1, o.ez);
-        const float4 v6 = add4(v2, o.ez);
-        const float4 v7 = add4(v3, o.ez);
-        box_take(mine, o.p);
-        box_take(mine, v1);
-        box_take(mine, v2);
-        box_take(mine, v3);
-        box_take(mine, v4);
-        box_take(mine, v5);
-        box_take(mine, v6);
-        box_take(mine, v7);
+    box_take(mine, o.p);
+    box_take(mine, v1);
+    box_take(mine, v2);
+    box_take(mine, v3);
+    box_take(mine, v4);
+    box_take(mine, v5);
+    box_take(mine, v6);
+    box_take(mine, v7);
+    const uint32_t idx = static_cast<uint32_t>(i - begin);
+    for (int c = 0; c < 3; ++c) {
+      if (acc.lo[c] > mine.lo[c]) {
+        acc.lo[c] = mine.lo[c];
+        acc.ilo[c] = idx;
+      }
+      if (acc.hi[c] < mine.hi[c]) {
+        acc.hi[c] = mine.hi[c];
+        acc.ihi[c] = idx;
       }
     }
-    mine = wave_box_reduce(mine);
-    // lane 0 holds this round's box; fold it into the running box in index order.
-    if (lane == 0) box_combine(acc, mine);
   }
-  if (lane == 0) partial[wave] = acc;
+  const Box6 box = box6_of(wave_box6i_reduce(acc));
+  if (lane == 0) partial[wave] = box;
 }
 
-__global__ __launch_bounds__(kWave) void bbox_final_kernel(const Box6* __restrict__ partial, size_t n_partial,
-                                                           float* __restrict__ out6)
+constexpr unsigned kBoxFinalThreads = 256;
+
+// One workgroup folds the partial boxes.  Thread t takes partials t, t + 256,
+// ... (coalesced loads, eight per round issued together) in order, keeping the
+// index of the partial that set each extremum; waves and then the workgroup's
+// four wave boxes are reduced by value and then partial index (partials are in
+// index order of their chunks).
+__global__ __launch_bounds__(kBoxFinalThreads) void bbox_final_kernel(const Box6* __restrict__ partial,
+                                                                      size_t n_partial, float* __restrict__ out6)
 {
-  const unsigned lane = threadIdx.x;
-  const size_t per = (n_partial + kWave - 1) / kWave;
-  Box6 acc = box_empty();
-  for (size_t k = lane * per; k < (lane + 1) * per && k < n_partial; ++k) box_combine(acc, partial[k]);
-  acc = wave_box_reduce(acc);
-  if (lane == 0) {
+  __shared__ Box6i waves[kBoxFinalThreads / kWave];
+  const unsigned t = threadIdx.x, lane = t & (kWave - 1);
+  Box6i run = box6i_empty();
+  // eight partials per thread and round, all loaded before any is folded (a
+  // clamped index past the end loads the last partial again, which is skipped)
+  constexpr unsigned kU = 8;
+  for (size_t k0 = t; k0 < n_partial; k0 += kU * kBoxFinalThreads) {
+    Box6 b[kU];
+#pragma unroll
+    for (unsigned u = 0; u < kU; ++u) {
+      const size_t k = k0 + u * kBoxFinalThreads;
+      b[u] = partial[k < n_partial ? k : n_partial - 1];
+    }
+#pragma unroll
+    for (unsigned u = 0; u < kU; ++u) {
+      // a slot past the end folds the empty box (it can change no value)
+      const size_t k = k0 + u * kBoxFinalThreads;
+      const bool valid = k < n_partial;
+      for (int c = 0; c < 3; ++c) {
+        take_lower(run.lo[c], run.ilo[c], valid ? b[u].lo[c] : FLT_MAX, static_cast<uint32_t>(k));
+        take_higher(run.hi[c], run.ihi[c], valid ? b[u].hi[c] : -FLT_MAX, static_cast<uint32_t>(k));
+      }
+    }
+  }
+  const Box6i wb = wave_box6i_reduce(run);
+  if (lane == 0) waves[t / kWave] = wb;
+  __syncthreads();
+  if (t != 0) return;
+  Box6i all = waves[0];
+  for (unsigned w = 1; w < kBoxFinalThreads / kWave; ++w)
+    for (int c = 0; c < 3; ++c) {
+      take_lower(all.lo[c], all.ilo[c], waves[w].lo[c], waves[w].ilo[c]);
+      take_higher(all.hi[c], all.ihi[c], waves[w].hi[c], waves[w].ihi[c]);
+    }
+  const Box6 acc = box6_of(all);
+  {
     // return Box3f(lower.xyz, upper.xyz): Boxnt(p0, p1) runs init(), update(p0), update(p1)
     // (dp/math/Boxnt.h:213-219), which turns the empty box into (-FLT_MAX, +FLT_MAX).
     Box6 r = box_empty();
@@ -308,9 +400,9 @@ __global__ __launch_bounds__(kWave) void bbox_final_kernel(const Box6* __restric
 
 size_t bbox_waves(size_t n)
 {
-  // At most 16 waves per CU of partials, at least 64 objects each.
+  // kBoxWavesPerCu waves per CU of partials at most, at least 64 objects each.
   size_t waves = (n + kWave - 1) / kWave;
-  const size_t cap = static_cast<size_t>(device_cu_count()) * 16;
+  const size_t cap = static_cast<size_t>(device_cu_count()) * kBoxWavesPerCu;
   if (waves > cap) waves = cap;
   if (waves == 0) waves = 1;
   waves = (waves + kBoxWavesPerBlock - 1) / kBoxWavesPerBlock * kBoxWavesPerBlock;
@@ -589,7 +681,7 @@ int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
                      reinterpret_cast<const float4*>(d_planes), ld, chunk, partial);
   int rc = check_launch("bbox_partial_kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kWave), 0, as_stream(stream), partial, waves, d_out6);
+  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kBoxFinalThreads), 0, as_stream(stream), partial, waves, d_out6);
   return check_launch("bbox_final_kernel");
 }
 

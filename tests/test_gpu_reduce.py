@@ -76,6 +76,31 @@ def test_bbox_empty_and_signed_zero(orc):
     assert got[0] == 0 and np.signbit(got[0])
 
 
+@pytest.mark.parametrize("n,ties,seed", [(300_000, 200, 1), (300_000, 3, 2), (2_000_000, 50, 3), (5_000, 40, 4)])
+def test_bbox_signed_zero_ties_across_lanes_and_waves(orc, n, ties, seed):
+    """Many objects whose lowest x is exactly +0 or -0, scattered over the whole
+    array (different lanes, waves and partial boxes): the box keeps the zero of
+    the first such object in index order, as the reference's sequential update."""
+    rng = np.random.default_rng(seed)
+    ID = np.eye(4, dtype=F)
+    mb = ID.copy()
+    mb[3, 0] = -0.0  # keeps the sign of lower.x: ((lx + -0) + -0) + -0 = lx for lx = +-0
+    lower = np.tile(np.array([[5, 5, 5]], F), (n, 1)) + rng.uniform(0, 100, (n, 3)).astype(F)
+    upper = lower + 1
+    at = np.sort(rng.choice(n, ties, replace=False))
+    signs = rng.integers(0, 2, ties)
+    lower[at] = np.stack([np.where(signs == 1, F(-0.0), F(0.0)), np.full(ties, -1, F), np.full(ties, -1, F)], 1)
+    upper[at] = (1, 0, 0)
+    xform = np.zeros(n, np.uint32)
+    xform[at] = 1
+    mats = np.stack([ID, mb])
+    got = gpu_bbox(lower, upper, mats, xform)
+    l4, e4 = scenes.extents(lower, upper)
+    want = orc.bbox_group(l4, e4, xform, mats)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    assert got[0] == 0 and bool(np.signbit(got[0])) == bool(signs[0])
+
+
 @pytest.mark.parametrize("n", [1, 31, 64, 65, 1000, 16_385, 2_000_000])
 def test_result_diff_matches_oracle(orc, n):
     lib = _lib.load()
