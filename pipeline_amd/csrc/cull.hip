@@ -499,6 +499,36 @@ __global__ __launch_bounds__(kDiffThreads) void diff_write_kernel(unsigned long 
   if (update_old && w < n_words) old_w[w] = new_w[w];
 }
 
+// The write pass for at most kDiffDirectBlocks blocks: each block sums the
+// counts of the blocks before it itself (no separate scan launch); the last
+// block writes the grand total.
+constexpr size_t kDiffDirectBlocks = 1024;
+
+__global__ __launch_bounds__(kDiffThreads) void diff_write_direct_kernel(unsigned long long* __restrict__ old_w,
+                                                                         const unsigned long long* __restrict__ new_w,
+                                                                         size_t n_words, size_t n_bits,
+                                                                         const uint32_t* __restrict__ block_count,
+                                                                         uint32_t* __restrict__ out, int update_old,
+                                                                         uint32_t* __restrict__ d_count)
+{
+  __shared__ uint32_t lds[kDiffThreads / kWave + 1];
+  uint32_t before = 0;  // this thread's share of the counts of blocks [0, blockIdx.x)
+  for (unsigned k = threadIdx.x; k < blockIdx.x; k += kDiffThreads) before += block_count[k];
+  uint32_t offset;
+  block_exclusive_scan(before, lds, &offset);
+  const size_t w = static_cast<size_t>(blockIdx.x) * kDiffThreads + threadIdx.x;
+  unsigned long long x = diff_word(old_w, new_w, w, n_words, n_bits);
+  uint32_t total;
+  uint32_t pos = offset + block_exclusive_scan(__popcll(x), lds, &total);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_count = offset + total;
+  while (x) {
+    const unsigned b = __ffsll(static_cast<long long>(x)) - 1;
+    out[pos++] = static_cast<uint32_t>(w * 64 + b);
+    x &= x - 1;
+  }
+  if (update_old && w < n_words) old_w[w] = new_w[w];
+}
+
 unsigned grid_for(size_t work, unsigned per_block)
 {
   return static_cast<unsigned>((work + per_block - 1) / per_block);
@@ -709,6 +739,11 @@ int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint3
                      counts);
   int rc = check_launch("diff_count_kernel");
   if (rc) return rc;
+  if (blocks <= kDiffDirectBlocks) {
+    hipLaunchKernelGGL(diff_write_direct_kernel, dim3(blocks), dim3(kDiffThreads), 0, as_stream(stream), o, nw,
+                       n_words, n_bits, counts, d_changed_idx, update_old, d_count);
+    return check_launch("diff_write_direct_kernel");
+  }
   hipLaunchKernelGGL(diff_scan_kernel, dim3(1), dim3(kDiffThreads), 0, as_stream(stream), counts,
                      static_cast<size_t>(blocks), d_count);
   rc = check_launch("diff_scan_kernel");

@@ -101,7 +101,7 @@ def test_bbox_signed_zero_ties_across_lanes_and_waves(orc, n, ties, seed):
     assert got[0] == 0 and bool(np.signbit(got[0])) == bool(signs[0])
 
 
-@pytest.mark.parametrize("n", [1, 31, 64, 65, 1000, 16_385, 2_000_000])
+@pytest.mark.parametrize("n", [1, 31, 64, 65, 1000, 16_385, 2_000_000, 20_000_000])
 def test_result_diff_matches_oracle(orc, n):
     lib = _lib.load()
     rng = np.random.default_rng(n)
