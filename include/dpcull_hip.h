@@ -258,7 +258,7 @@ int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint3
 /* ---- group bounding box (ManagerBitSet::calculateBoundingBox, dp/culling/src/ManagerBitSet.cpp:268-306)
  * World-space AABB of every object's eight transformed corners; d_out6 =
  * lower.xyz, upper.xyz as the reference's Box3f(min, max) returns them.
- * d_workspace needs dpc_bbox_workspace(n) bytes. */
+ * d_workspace needs dpc_bbox_workspace(n) bytes, 8-byte aligned. */
 size_t dpc_bbox_workspace(size_t n_objects);
 int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
              const float* d_planes, size_t ld, float* d_out6, void* d_workspace, void* stream);

@@ -187,7 +187,7 @@ __global__ void boxes_kernel(const float* __restrict__ lu, size_t n, float4* __r
 // selection, which is associative over ordered ranges: combining a range with the
 // range that follows it, left operand first, reproduces the sequential scan bit for
 // bit (including which of +0/-0 survives and that NaN never enters).
-struct Box6 {
+struct __align__(8) Box6 {  // (8-byte aligned: a partial box loads as three dwordx2)
   float lo[3], hi[3];
 };
 
@@ -288,8 +288,10 @@ __device__ __forceinline__ Box6 box6_of(const Box6i& b)
 // Each wave scans one contiguous chunk of objects, lane l taking objects
 // begin + l, begin + l + 64, ... (coalesced loads).  A lane first takes the box
 // of its object's eight corners, then folds it into its running box with the
-// object's index; one wave reduction per chunk gives the chunk's box.  Partials
-// are stored in wave order, i.e. in index order of their chunks.
+// object's index; one wave reduction per chunk gives the chunk's box.  The
+// workgroup's waves hold consecutive chunks, so folding their boxes in wave
+// order with strict comparisons keeps the first index among equal extrema; one
+// partial per workgroup is stored, in index order of the chunks.
 __global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4* __restrict__ box_a,
                                                                    const float2* __restrict__ box_b,
                                                                    const uint32_t* __restrict__ xform, size_t n,
@@ -336,14 +338,24 @@ __global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4*
       }
     }
   }
+  __shared__ Box6 waves[kBoxWavesPerBlock];
   const Box6 box = box6_of(wave_box6i_reduce(acc));
-  if (lane == 0) partial[wave] = box;
+  if (lane == 0) waves[threadIdx.x / kWave] = box;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  Box6 all = waves[0];
+  for (unsigned w = 1; w < kBoxWavesPerBlock; ++w)
+    for (int c = 0; c < 3; ++c) {
+      if (waves[w].lo[c] < all.lo[c]) all.lo[c] = waves[w].lo[c];
+      if (waves[w].hi[c] > all.hi[c]) all.hi[c] = waves[w].hi[c];
+    }
+  partial[blockIdx.x] = all;
 }
 
 constexpr unsigned kBoxFinalThreads = 256;
 
 // One workgroup folds the partial boxes.  Thread t takes partials t, t + 256,
-// ... (coalesced loads, eight per round issued together) in order, keeping the
+// ... (coalesced loads, four per round issued together) in order, keeping the
 // index of the partial that set each extremum; waves and then the workgroup's
 // four wave boxes are reduced by value and then partial index (partials are in
 // index order of their chunks).
@@ -353,9 +365,9 @@ __global__ __launch_bounds__(kBoxFinalThreads) void bbox_final_kernel(const Box6
   __shared__ Box6i waves[kBoxFinalThreads / kWave];
   const unsigned t = threadIdx.x, lane = t & (kWave - 1);
   Box6i run = box6i_empty();
-  // eight partials per thread and round, all loaded before any is folded (a
+  // four partials per thread and round, all loaded before any is folded (a
   // clamped index past the end loads the last partial again, which is skipped)
-  constexpr unsigned kU = 8;
+  constexpr unsigned kU = 4;
   for (size_t k0 = t; k0 < n_partial; k0 += kU * kBoxFinalThreads) {
     Box6 b[kU];
 #pragma unroll
@@ -689,12 +701,13 @@ int dpc_boxes_from_lower_upper(const float* d_lower_upper, size_t n, float* d_bo
   return check_launch("boxes_kernel");
 }
 
-size_t dpc_bbox_workspace(size_t n_objects) { return bbox_waves(n_objects) * sizeof(Box6); }
+size_t dpc_bbox_workspace(size_t n_objects) { return bbox_waves(n_objects) / kBoxWavesPerBlock * sizeof(Box6); }
 
 int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
              const float* d_planes, size_t ld, float* d_out6, void* d_workspace, void* stream)
 {
-  if (!d_out6 || !d_workspace) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bbox: null output/workspace");
+  if (!d_out6 || !d_workspace || !aligned(d_workspace, 8))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bbox: null output/workspace (or workspace not 8-byte aligned)");
   if (n_objects && (!d_box_a || !d_box_b || !d_planes || !aligned(d_box_a, 16) || !aligned(d_box_b, 8) ||
                     !aligned(d_planes, 16)))
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_bbox: bad object/matrix pointer");
@@ -711,7 +724,8 @@ int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
                      reinterpret_cast<const float4*>(d_planes), ld, chunk, partial);
   int rc = check_launch("bbox_partial_kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kBoxFinalThreads), 0, as_stream(stream), partial, waves, d_out6);
+  hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kBoxFinalThreads), 0, as_stream(stream), partial,
+                     waves / kBoxWavesPerBlock, d_out6);
   return check_launch("bbox_final_kernel");
 }
 
