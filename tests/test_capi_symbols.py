@@ -29,3 +29,24 @@ def test_binding_matches_header():
 
 def test_abi_version_without_gpu(lib):
     assert lib.dpc_abi_version() == 1
+
+
+def test_argument_errors_are_reported_before_any_device_call(lib):
+    """Bad arguments come back as DPC_ERR_INVALID_ARGUMENT with a message, from
+    host-side checks that run before the device is touched (the addresses below
+    are never dereferenced)."""
+    inval = 1  # DPC_ERR_INVALID_ARGUMENT
+    a = 0x10000  # 256-byte aligned
+    # dpc_bbox: workspace not 8-byte aligned, box_b not 8-byte aligned, ld < n without indices
+    assert lib.dpc_bbox(a, a, None, 10, a, 10, a, a + 4, None) == inval
+    assert b"aligned" in lib.dpc_last_error()
+    assert lib.dpc_bbox(a, a + 4, None, 10, a, 10, a, a, None) == inval
+    assert lib.dpc_bbox(a, a, None, 10, a, 5, a, a, None) == inval
+    assert b"ld >= n" in lib.dpc_last_error()
+    # dpc_cull: no view-projection matrix
+    assert lib.dpc_cull(a, a, None, 10, a, 10, None, a, None) == inval
+    # dpc_result_diff: no count output
+    assert lib.dpc_result_diff(a, a, 100, a, None, 0, a, None) == inval
+    # dpc_inverse_transpose: more matrices than the leading dimension holds
+    assert lib.dpc_inverse_transpose(a, 5, None, 10, a, 10, None, None) == inval
+    assert b"exceeds" in lib.dpc_last_error()
