@@ -1,5 +1,6 @@
 """The C-ABI library loads (no GPU needed) and exports every function that
-include/dpcull_hip.h declares; the Python binding declares the same set."""
+include/dpcull_hip.h declares; the Python binding declares the same set; bad
+arguments are rejected on the host without a GPU."""
 import re
 
 from conftest import ROOT
