@@ -54,8 +54,8 @@ def parse():
     ap.add_argument("--skip-extra", action="store_true", help="only the batched and per-view lines")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-probe", action="store_true",
-                    help="launch only a 512 MiB device copy, the batched kernel, the flat kernel and the tree "
-                         "propagation a few times each (for rocprofv3 --pmc passes); prints no JSON")
+                    help="launch only a 512 MiB device copy, the batched kernel, the flat kernel, the tree "
+                         "propagation and the bounding box a few times each (for rocprofv3 --pmc passes); prints no JSON")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
     ap.add_argument("--serve-streams", type=int, default=8, help="streams of the serving measurement")
     ap.add_argument("--serve-depth", type=int, default=2, help="batches in flight per stream in the serving measurement")
@@ -421,8 +421,9 @@ def measured_traffic(kernel: str):
 def pmc_probe():
     """Kernels for the counter passes: a plain device copy of PMC_COPY_BYTES
     (calibrates FETCH_SIZE / WRITE_SIZE to bytes), then the batched launch of
-    the headline workload, the flat single-frustum launch and the propagation
-    of the 1,000,000-node tree, 5 times each."""
+    the headline workload, the flat single-frustum launch, the propagation of
+    the 1,000,000-node tree and the group bounding box of its 2,372,872
+    objects, 5 times each."""
     from pipeline_amd import _lib, scenes
     from pipeline_amd import device as dev
     from pipeline_amd import autotune
@@ -474,6 +475,16 @@ def pmc_probe():
     for _ in range(5):
         _lib.check(lib.dpc_tree_propagate_plan(tplan, d_local.ptr, d_world.ptr, None, d_dirty.ptr, stream.handle))
     stream.sync()
+    # group bounding box of the tree's 2,372,872 objects over the propagated world matrices
+    objs = dev.upload_scene(ts.obj_lower, ts.obj_upper, np.eye(4, dtype=np.float32)[None], ts.obj_slot)
+    ws = dev.DeviceArray(max(lib.dpc_bbox_workspace(objs.n_objects), 8), np.uint8)
+    box6 = dev.DeviceArray(8, np.float32)
+    for _ in range(5):
+        _lib.check(lib.dpc_bbox(objs.box_a.ptr, objs.box_b.ptr, objs.xform_addr, objs.n_objects, d_world.ptr,
+                                ts.n_slots, box6.ptr, ws.ptr, stream.handle))
+    stream.sync()
+    for d in (objs, ws, box6):
+        d.free()
     lib.dpc_tree_plan_destroy(tplan)
     for d in (d_local, d_world, d_dirty):
         d.free()
