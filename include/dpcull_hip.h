@@ -218,7 +218,8 @@ int dpc_tree_propagate(const float* d_local, float* d_world, size_t ld, const ui
  * runs (slots per step).  Every tile run is checked against the properties
  * its kernel relies on before anything is uploaded (DPC_ERR_INTERNAL if one
  * fails).  dpc_tree_propagate_plan has dpc_tree_propagate's semantics for
- * that layout. */
+ * that layout.  n_levels = 0 (a tree holding only its root) gives a plan that
+ * launches nothing. */
 typedef struct dpc_tree_plan dpc_tree_plan;
 int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_t* h_level_offsets, size_t n_levels,
                          uint32_t tile_slots, dpc_tree_plan** plan);  /* tile_slots: 256, 512, 1024 or 0 (1024) */

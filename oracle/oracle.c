@@ -11,6 +11,7 @@
 
 #include <float.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <string.h>
 
 /* dp/math/Matmnt.h:1371-1379 */
@@ -330,13 +331,49 @@ static void* cull_range(void* arg)
   return NULL;
 }
 
+/* The host threads: created on first use and kept for the life of the process,
+ * so a call costs a wake-up and a join, not thread creation (the CPU timing
+ * then measures the cores).  Worker k (1 <= k < threads) runs part k of each
+ * job; the calling thread runs part 0. */
+enum { kMaxThreads = 256 };
+static struct {
+  pthread_mutex_t mu;
+  pthread_cond_t go, done;
+  int workers;             /* threads created (ids 1..workers) */
+  unsigned long job;       /* bumped for every job */
+  int pending;             /* workers still inside the current job */
+  void* (*fn)(void*);
+  orc_range* parts;
+  int n_parts;
+  unsigned long first_job[kMaxThreads]; /* the job a worker was created for: it runs that one first */
+} g_pool = {PTHREAD_MUTEX_INITIALIZER, PTHREAD_COND_INITIALIZER, PTHREAD_COND_INITIALIZER, 0, 0, 0, NULL, NULL, 0, {0}};
+
+static void* pool_worker(void* arg)
+{
+  const int id = (int)(intptr_t)arg;
+  unsigned long seen = 0;
+  pthread_mutex_lock(&g_pool.mu);
+  seen = g_pool.first_job[id] - 1;
+  for (;;) {
+    while (g_pool.job == seen) pthread_cond_wait(&g_pool.go, &g_pool.mu);
+    seen = g_pool.job;
+    void* (*fn)(void*) = g_pool.fn;
+    orc_range* part = id < g_pool.n_parts ? &g_pool.parts[id] : NULL;
+    pthread_mutex_unlock(&g_pool.mu);
+    if (part) fn(part);
+    pthread_mutex_lock(&g_pool.mu);
+    if (--g_pool.pending == 0) pthread_cond_signal(&g_pool.done);
+  }
+  return NULL;
+}
+
+static pthread_mutex_t g_run_mu = PTHREAD_MUTEX_INITIALIZER; /* one job at a time */
+
 static void run_ranges(size_t n, int threads, orc_range proto, void* (*fn)(void*))
 {
-  enum { kMaxThreads = 256 };
+  pthread_mutex_lock(&g_run_mu);
   if (threads < 1) threads = 1;
   if (threads > kMaxThreads) threads = kMaxThreads;
-  pthread_t tid[kMaxThreads];
-  int created[kMaxThreads];
   orc_range part[kMaxThreads];
   const size_t words = (n + 63) / 64;
   const size_t per = (words + (size_t)threads - 1) / (size_t)threads;
@@ -347,13 +384,29 @@ static void run_ranges(size_t n, int threads, orc_range proto, void* (*fn)(void*
     if (part[t].begin > n) part[t].begin = n;
     if (part[t].end > n) part[t].end = n;
   }
-  for (int t = 1; t < threads; ++t) {
-    created[t] = pthread_create(&tid[t], NULL, fn, &part[t]) == 0;
-    if (!created[t]) fn(&part[t]);  /* no thread: do its part here */
+  pthread_mutex_lock(&g_pool.mu);
+  while (g_pool.workers < threads - 1) {
+    pthread_t tid;
+    g_pool.first_job[g_pool.workers + 1] = g_pool.job + 1;  /* the job posted below */
+    if (pthread_create(&tid, NULL, pool_worker, (void*)(intptr_t)(g_pool.workers + 1)) != 0) break;
+    pthread_detach(tid);
+    ++g_pool.workers;
   }
+  /* parts no worker exists for (thread creation failed) run here */
+  const int served = g_pool.workers + 1 < threads ? g_pool.workers + 1 : threads;
+  g_pool.fn = fn;
+  g_pool.parts = part;
+  g_pool.n_parts = served;
+  g_pool.pending = g_pool.workers;
+  ++g_pool.job;
+  pthread_cond_broadcast(&g_pool.go);
+  pthread_mutex_unlock(&g_pool.mu);
   fn(&part[0]);
-  for (int t = 1; t < threads; ++t)
-    if (created[t]) pthread_join(tid[t], NULL);
+  for (int t = served; t < threads; ++t) fn(&part[t]);
+  pthread_mutex_lock(&g_pool.mu);
+  while (g_pool.pending > 0) pthread_cond_wait(&g_pool.done, &g_pool.mu);
+  pthread_mutex_unlock(&g_pool.mu);
+  pthread_mutex_unlock(&g_run_mu);
 }
 
 void orc_obb_build_group_mt(size_t n, const float* lower4, const float* extent4, const uint64_t* xform,

@@ -83,20 +83,5 @@ def build(verbose: bool = False) -> Path:
     return out
 
 
-def build_cpp_tests() -> None:
-    """The C++ tests of the host mirrors (tests/cpp), linked against the library."""
-    _run(["make", "-s", "-C", str(ROOT / "tests" / "cpp"), "-j4"])
-
-
-def build_oracle() -> None:
-    """Compile the CPU oracle (test infrastructure) and, where the reference
-    sources are present, its compiled-reference companion oracle/_ref."""
-    _run(["make", "-s", "-C", str(ROOT / "oracle"), "liboracle.so"])
-    if Path(os.environ.get("DPC_REFERENCE", "/root/reference")).joinpath("dp").is_dir():
-        _run(["make", "-s", "-C", str(ROOT / "oracle"), "ref",
-              f"REF={os.environ.get('DPC_REFERENCE', '/root/reference')}"])
-
-
 if __name__ == "__main__":
     build(verbose=True)
-    build_oracle()

@@ -427,8 +427,11 @@ class ManagerImpl : public culling::Manager {
   {
     const size_t n = g.m_objects.size();
     g.xform_slot.reserve(std::max<size_t>(n, 1));
+    // The device slot table covers the tree's layout at its last compute
+    // (deviceLd entries); an index added since then maps to no slot, and the
+    // object is treated like one whose matrix index lies outside the array.
     if (n)
-      check(dpc_gather_u32(g.m_tree->deviceSlotOfIndex(), g.m_tree->getTransformCount(), g.xform.p, n, g.xform_slot.p,
+      check(dpc_gather_u32(g.m_tree->deviceSlotOfIndex(), g.m_tree->deviceLd(), g.xform.p, n, g.xform_slot.p,
                            m_stream),
             "map indices to tree slots");
   }

@@ -190,7 +190,7 @@ TreeLayout::Compiled TreeLayout::compile() const
 struct Tree::Device {
   void* stream = nullptr;
   size_t ld = 0;
-  DevBuf<float> local, world, world_tmp, aos, it;
+  DevBuf<float> local, world, aos, it;
   DevBuf<uint32_t> parent_slot, index_of_slot, slot_of_index, old_slot, idx, bits;
   DevBuf<uint8_t> dirty_local, dirty_world;
   DevBuf<float> delta;
@@ -251,57 +251,76 @@ void Tree::updateLocalMatrix(Index index, Mat44f const& matrix)
 
 void Tree::recompile()
 {
+  // The new layout is built in buffers of its own and the tree takes it over
+  // only when every step succeeded: a failure (an exception) leaves the tree
+  // with its previous layout, worlds and plan, as if recompile had not run.
   Device& d = *m_dev;
   TreeLayout::Compiled next = m_layout.compile();
   const size_t cap = m_layout.capacity();
   const size_t old_ld = d.ld;
   const bool had_world = old_ld != 0;
-  d.local.reserve(4 * 4 * cap);
-  d.world_tmp.reserve(4 * 4 * cap);
-  d.aos.reserve(16 * cap);
-  d.parent_slot.reserve(cap);
-  d.index_of_slot.reserve(cap);
-  d.slot_of_index.reserve(cap);
+  DevBuf<float> local, world;
+  DevBuf<uint32_t> parent_slot, index_of_slot, slot_of_index, bits;
+  DevBuf<uint8_t> dirty_local, dirty_world;
+  local.reserve(4 * 4 * cap);
+  world.reserve(4 * 4 * cap);
+  parent_slot.reserve(cap);
+  index_of_slot.reserve(cap);
+  slot_of_index.reserve(cap);
+  dirty_local.reserve(cap);
+  dirty_world.reserve(cap + 4);
+  bits.reserve(2 * ((cap + 63) / 64));
+  d.aos.reserve(16 * cap);  // staging only: its contents are never kept
   d.old_slot.reserve(cap);
-  d.dirty_local.reserve(cap);
-  d.dirty_world.reserve(cap + 4);
-  d.bits.reserve(2 * ((cap + 63) / 64));
 
   // World matrices keep their values across the relayout: new slot s takes the
   // old slot of the same index (indices beyond the old capacity start at zero).
   std::vector<Mat44f> local_by_slot(cap);
   for (size_t s = 0; s < cap; ++s) local_by_slot[s] = m_local[next.index_of_slot[s]];
-  check(dpc_memcpy_h2d(d.aos.p, local_by_slot.data(), cap * sizeof(Mat44f), d.stream), "upload locals");
-  check(dpc_matrices_from_aos(d.aos.p, 64, cap, d.local.p, cap, d.stream), "locals to planes");
   std::vector<uint32_t> old_slot;  // host buffers stay alive until the stream is synchronised below
   std::vector<Mat44f> w;
-  if (had_world) {
-    old_slot.resize(cap);
-    for (size_t s = 0; s < cap; ++s) {
-      const uint32_t i = next.index_of_slot[s];
-      old_slot[s] = i < m_compiled.slot_of_index.size() ? m_compiled.slot_of_index[i] : 0xFFFFFFFFu;
+  try {
+    check(dpc_memcpy_h2d(d.aos.p, local_by_slot.data(), cap * sizeof(Mat44f), d.stream), "upload locals");
+    check(dpc_matrices_from_aos(d.aos.p, 64, cap, local.p, cap, d.stream), "locals to planes");
+    if (had_world) {
+      old_slot.resize(cap);
+      for (size_t s = 0; s < cap; ++s) {
+        const uint32_t i = next.index_of_slot[s];
+        old_slot[s] = i < m_compiled.slot_of_index.size() ? m_compiled.slot_of_index[i] : 0xFFFFFFFFu;
+      }
+      check(dpc_memcpy_h2d(d.old_slot.p, old_slot.data(), cap * sizeof(uint32_t), d.stream), "upload slot map");
+      check(dpc_matrices_to_aos(d.world.p, old_ld, d.old_slot.p, cap, d.aos.p, 64, d.stream), "gather worlds");
+    } else {
+      w.assign(cap, Mat44f{});
+      w[0] = Mat44f::identity();
+      check(dpc_memcpy_h2d(d.aos.p, w.data(), cap * sizeof(Mat44f), d.stream), "upload worlds");
     }
-    check(dpc_memcpy_h2d(d.old_slot.p, old_slot.data(), cap * sizeof(uint32_t), d.stream), "upload slot map");
-    check(dpc_matrices_to_aos(d.world.p, old_ld, d.old_slot.p, cap, d.aos.p, 64, d.stream), "gather worlds");
-  } else {
-    w.assign(cap, Mat44f{});
-    w[0] = Mat44f::identity();
-    check(dpc_memcpy_h2d(d.aos.p, w.data(), cap * sizeof(Mat44f), d.stream), "upload worlds");
+    check(dpc_matrices_from_aos(d.aos.p, 64, cap, world.p, cap, d.stream), "worlds to planes");
+    check(dpc_memcpy_h2d(parent_slot.p, next.parent_slot.data(), cap * sizeof(uint32_t), d.stream), "parents");
+    check(dpc_memcpy_h2d(index_of_slot.p, next.index_of_slot.data(), cap * sizeof(uint32_t), d.stream), "slots");
+    check(dpc_memcpy_h2d(slot_of_index.p, next.slot_of_index.data(), cap * sizeof(uint32_t), d.stream), "slots");
+    check(dpc_fill_u32(reinterpret_cast<uint32_t*>(dirty_world.p), 0u, (cap + 4) / 4, d.stream), "clear dirty");
+  } catch (...) {
+    (void)dpc_stream_sync(d.stream);  // the host buffers above must outlive the copies already queued
+    throw;
   }
-  check(dpc_matrices_from_aos(d.aos.p, 64, cap, d.world_tmp.p, cap, d.stream), "worlds to planes");
-  d.world.swap(d.world_tmp);
-  d.world_tmp.reserve(4 * 4 * cap);
-  check(dpc_memcpy_h2d(d.parent_slot.p, next.parent_slot.data(), cap * sizeof(uint32_t), d.stream), "parents");
-  check(dpc_memcpy_h2d(d.index_of_slot.p, next.index_of_slot.data(), cap * sizeof(uint32_t), d.stream), "slots");
-  check(dpc_memcpy_h2d(d.slot_of_index.p, next.slot_of_index.data(), cap * sizeof(uint32_t), d.stream), "slots");
-  check(dpc_fill_u32(reinterpret_cast<uint32_t*>(d.dirty_world.p), 0u, (cap + 4) / 4, d.stream), "clear dirty");
   check(dpc_stream_sync(d.stream), "recompile sync");
-  d.ld = cap;
-  if (d.plan) dpc_tree_plan_destroy(d.plan);
-  d.plan = nullptr;
+  dpc_tree_plan* plan = nullptr;
   check(dpc_tree_plan_create(next.parent_slot.data(), cap, next.level_offsets.data(), next.level_offsets.size() - 1, 0,
-                             &d.plan),
+                             &plan),
         "dpc_tree_plan_create");
+  // commit: nothing below can fail
+  if (d.plan) dpc_tree_plan_destroy(d.plan);
+  d.plan = plan;
+  d.local.swap(local);
+  d.world.swap(world);
+  d.parent_slot.swap(parent_slot);
+  d.index_of_slot.swap(index_of_slot);
+  d.slot_of_index.swap(slot_of_index);
+  d.dirty_local.swap(dirty_local);
+  d.dirty_world.swap(dirty_world);
+  d.bits.swap(bits);
+  d.ld = cap;
   m_compiled = std::move(next);
   m_structureDirty = false;
   m_itAll = true;  // slots moved: recompute every inverse transpose from the relaid-out worlds

@@ -721,7 +721,7 @@ int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_
 {
   if (!out) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: null plan pointer");
   *out = nullptr;
-  if (!h_parent_slot || !h_level_offsets || n_levels == 0 || ld == 0 || ld > 0xFFFFFFFFull)
+  if (!h_parent_slot || !h_level_offsets || ld == 0 || ld > 0xFFFFFFFFull)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: bad arguments");
   for (size_t l = 0; l < n_levels; ++l)
     if (h_level_offsets[l] > h_level_offsets[l + 1])
@@ -730,6 +730,15 @@ int dpc_tree_plan_create(const uint32_t* h_parent_slot, size_t ld, const uint64_
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: levels end beyond ld");
   if (tile_slots != 0 && tile_slots != 256 && tile_slots != 512 && tile_slots != 1024)
     return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_tree_plan_create: tile_slots must be 0, 256, 512 or 1024");
+  if (n_levels == 0) {
+    // a tree that holds only its root: propagation has nothing to compute
+    // (Tree::compute visits no level and notifies an empty change set)
+    auto plan = std::make_unique<dpc_tree_plan>();
+    plan->ld = ld;
+    plan->threads = tile_slots ? tile_slots : kDefaultTileThreads;
+    *out = plan.release();
+    return DPC_OK;
+  }
   auto plan = std::make_unique<dpc_tree_plan>();
   plan->ld = ld;
   plan->threads = tile_slots ? tile_slots : kDefaultTileThreads;

@@ -27,14 +27,6 @@ def orc():
 
 
 @pytest.fixture(scope="session")
-def ref():
-    from oracle import oracle
-    if not oracle.ref_available():
-        pytest.skip("oracle/_ref (the compiled reference) is not built here")
-    return oracle.load_ref()
-
-
-@pytest.fixture(scope="session")
 def lib():
     from pipeline_amd import _lib
     return _lib.load()

@@ -74,16 +74,20 @@ static void replay_reference_trace(const std::string& path)
       const TreeCompute& c = t.computes[op.a];
       const size_t nw = c.event.size();
       const Mat44f* w = tree.getWorldMatrices();
-      size_t checked = 0;
+      size_t checked = 0, kept = 0;
       for (size_t i = 0; i < nw; ++i) {
         EXPECT(rec.bit(i) == (c.event[i] != 0), "compute %u: event bit %zu", op.a, i);
-        if (c.event[i]) {
-          EXPECT(std::memcmp(w[i].m, &c.world[16 * i], 64) == 0, "compute %u: world matrix %zu differs", op.a, i);
-          ++checked;
-        }
+        // Every index below the reference's highest one has been allocated at
+        // some point (lowest-free allocation), so its world matrix is defined:
+        // recomputed now, or kept from an earlier compute across relayouts
+        // (live transforms that did not change, and removed ones whose
+        // orphaned children still read them).
+        EXPECT(std::memcmp(w[i].m, &c.world[16 * i], 64) == 0, "compute %u: world matrix %zu differs (%s)", op.a, i,
+               c.event[i] ? "recomputed" : "kept");
+        ++(c.event[i] ? checked : kept);
       }
       for (size_t i = nw; i < rec.count; ++i) EXPECT(!rec.bit(i), "event bit beyond the reference's indices");
-      std::printf("  reference compute %u: %zu recomputed matrices bit-identical\n", op.a, checked);
+      std::printf("  reference compute %u: %zu recomputed and %zu kept matrices bit-identical\n", op.a, checked, kept);
     }
   }
   EXPECT(rec.calls == static_cast<int>(t.computes.size()), "one event per compute");
@@ -176,10 +180,60 @@ static void random_frames()
   }
 }
 
+// A tree holding only its root computes nothing and notifies an empty change
+// set (Tree.cpp:133-166 visits no level); it must stay usable afterwards: add
+// transforms, compute, remove them all (root only again), compute, add again.
+static void root_only_tree()
+{
+  Tree tree;
+  Recorder rec;
+  tree.attach(&rec);
+  auto none_set = [&]() {
+    for (uint64_t w : rec.bits)
+      if (w) return false;
+    return true;
+  };
+  tree.compute();
+  EXPECT(rec.calls == 1 && none_set(), "root-only compute: one empty event");
+  EXPECT(std::memcmp(tree.getWorldMatrix(0).m, Mat44f::identity().m, 64) == 0, "root world is the identity");
+  Mat44f a = Mat44f::identity(), b = Mat44f::identity();
+  a[3][0] = 2.f;
+  a[0][1] = 0.5f;
+  b[3][2] = -3.f;
+  b[1][1] = 1.25f;
+  for (int round = 0; round < 2; ++round) {
+    const Index ia = tree.addTransform(0, a);
+    const Index ib = tree.addTransform(ia, b);
+    tree.compute();
+    // the oracle on the same two levels
+    const size_t cap = tree.getTransformCount();
+    std::vector<float> loc(16 * cap, 0.f), world(16 * cap, 0.f);
+    std::vector<uint8_t> dl(cap, 0), dw(cap, 0), ev(cap, 0);
+    std::memcpy(world.data(), Mat44f::identity().m, 64);
+    std::memcpy(&loc[16 * ia], a.m, 64);
+    std::memcpy(&loc[16 * ib], b.m, 64);
+    dl[ia] = dl[ib] = 1;
+    const uint64_t off[3] = {0, 1, 2};
+    const uint32_t entries[4] = {0, ia, ia, ib};
+    orc_tree_compute(2, off, entries, loc.data(), world.data(), cap, dl.data(), dw.data(), ev.data());
+    const Mat44f* w = tree.getWorldMatrices();
+    EXPECT(std::memcmp(w[ia].m, &world[16 * ia], 64) == 0 && std::memcmp(w[ib].m, &world[16 * ib], 64) == 0,
+           "round %d: world matrices after adding to a root-only tree", round);
+    EXPECT(rec.bit(ia) && rec.bit(ib) && !rec.bit(0), "round %d: event bits", round);
+    tree.removeTransform(ib);
+    tree.removeTransform(ia);
+    tree.compute();
+    EXPECT(none_set(), "round %d: root-only again: empty event", round);
+  }
+  EXPECT(rec.calls == 5, "one event per compute (%d)", rec.calls);
+  std::printf("  root-only tree: %d computes\n", rec.calls);
+}
+
 int main(int argc, char** argv)
 {
   const std::string path = argc > 1 ? argv[1] : "tests/golden/ref_tree_trace.bin";
   replay_reference_trace(path);
+  root_only_tree();
   random_frames();
   std::printf("tree gpu: %d failures\n", failures);
   return failures ? 1 : 0;
