@@ -22,6 +22,7 @@
 #include "cpu_manager.hpp"
 #include "dpcull/Manager.hpp"
 #include "dpcull/Tree.hpp"
+#include "self_front.hpp"
 
 using namespace dpcull;
 using namespace dpcull::culling;
@@ -168,11 +169,13 @@ struct World {
   std::vector<ObjectSharedPtr> objects;
 };
 
-static void random_trace(bool refresh, unsigned seed, int steps)
+static void random_trace(bool refresh, unsigned seed, int steps, bool via_adapter = false)
 {
   std::mt19937 rng(seed);
   std::uniform_real_distribution<float> u(-1.f, 1.f);
-  std::unique_ptr<Manager> hipm(hip::Manager::create(hip::Options{refresh}));
+  // the HIP manager directly, or behind the reference-interface adapter
+  std::unique_ptr<Manager> hipm(via_adapter ? static_cast<Manager*>(new AdapterManager(hip::Options{refresh}))
+                                            : hip::Manager::create(hip::Options{refresh}));
   oracle::CpuManager cpum(refresh);
   World w[2] = {{hipm.get(), {}, {}, {}}, {&cpum, {}, {}, {}}};
   const int G = 3, M = 64;
@@ -294,8 +297,9 @@ static void random_trace(bool refresh, unsigned seed, int steps)
              a.upper[0], b.lower[0], b.upper[0]);
     }
   }
-  std::printf("  random trace seed %u (%s): %zu culls, %zu changed objects, %zu objects\n", seed,
-              refresh ? "refresh" : "cpu-cache", culls, changes, member.size());
+  std::printf("  random trace seed %u (%s%s): %zu culls, %zu changed objects, %zu objects\n", seed,
+              refresh ? "refresh" : "cpu-cache", via_adapter ? ", through the adapter" : "", culls, changes,
+              member.size());
 }
 
 // ---------------------------------------------------------------------------
@@ -450,10 +454,18 @@ int main()
     scenario(*a, false, "hip/cpu-cache");
     scenario(*b, true, "hip/refresh");
   }
+  {
+    // the same hand-traced expectations through the reference-interface adapter
+    AdapterManager a(hip::Options{false}), b(hip::Options{true});
+    scenario(a, false, "adapter/cpu-cache");
+    scenario(b, true, "adapter/refresh");
+  }
   for (unsigned seed : {1u, 2u, 3u}) {
     random_trace(false, seed, 3000);
     random_trace(true, seed + 100, 3000);
   }
+  random_trace(false, 7, 3000, true);
+  random_trace(true, 107, 3000, true);
   batch_equals_sequence();
   tree_bound_group();
   std::printf("manager parity: %d failures\n", failures);

@@ -44,3 +44,13 @@ def test_gpu_tree_matches_reference_trace():
 def test_manager_matches_cpu_manager():
     out = run("manager_parity")
     assert "manager parity: 0 failures" in out
+
+
+@pytest.mark.gpu
+def test_xbar_integration_frames():
+    """The reference-interface adapter (integration/dpcull_adapter.hpp) over the
+    HIP manager, driven with xbar CullingImpl's call sequence and the tree
+    observer that forwards changed world matrices to groupMatrixChanged, against
+    the CPU manager restatement and a full cull of the current scene."""
+    out = run("integration_test")
+    assert "integration: 0 failures" in out
