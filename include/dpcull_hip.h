@@ -256,6 +256,20 @@ size_t dpc_result_diff_workspace(size_t n_bits);
 int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx,
                     uint32_t* d_count, int update_old, void* d_workspace, void* stream);
 
+/* dpc_result_diff with no workspace argument and no allocation of any kind.
+ * No reference counterpart: it is for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, hosts that forbid
+ * hipMalloc).  Its per-block counts live in a fixed 256 KiB device array of the
+ * library, one per device, held until the process ends; n_bits above
+ * DPC_RESULT_DIFF_NOALLOC_MAX_BITS (2^30) is rejected with
+ * DPC_ERR_INVALID_ARGUMENT.  Every call on one device shares that array, so
+ * calls that overlap on one device (concurrent streams, or threads with their
+ * own streams) race: serialise them, e.g. on one stream.  Results are those of
+ * dpc_result_diff. */
+#define DPC_RESULT_DIFF_NOALLOC_MAX_BITS 1073741824ull
+int dpc_result_diff_noalloc(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx,
+                            uint32_t* d_count, int update_old, void* stream);
+
 /* ---- group bounding box (ManagerBitSet::calculateBoundingBox, dp/culling/src/ManagerBitSet.cpp:268-306)
  * World-space AABB of every object's eight transformed corners; d_out6 =
  * lower.xyz, upper.xyz as the reference's Box3f(min, max) returns them.

@@ -122,6 +122,7 @@ _SIGNATURES = {
     "dpc_bytes_to_bits_indexed": (I, [P, P, SZ, P, SZ, P]),
     "dpc_result_diff_workspace": (SZ, [SZ]),
     "dpc_result_diff": (I, [P, P, SZ, P, P, I, P, P]),
+    "dpc_result_diff_noalloc": (I, [P, P, SZ, P, P, I, P]),
     "dpc_bbox_workspace": (SZ, [SZ]),
     "dpc_bbox": (I, [P, P, P, SZ, P, SZ, P, P, P]),
     "dpc_inverse_transpose": (I, [P, SZ, P, SZ, P, SZ, P, P]),

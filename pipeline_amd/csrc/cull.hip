@@ -7,6 +7,7 @@
 // and ResultBitSet::updateChanged (dp/culling/src/ResultBitSet.cpp:61-108).
 #include <atomic>
 #include <cfloat>
+#include <mutex>
 
 #include "cull_math.hpp"
 #include "internal.hpp"
@@ -736,17 +737,31 @@ size_t dpc_result_diff_workspace(size_t n_bits)
   return (blocks ? blocks : 1) * sizeof(uint32_t);
 }
 
-int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx, uint32_t* d_count,
-                    int update_old, void* d_workspace, void* stream)
+}  // extern "C"
+
+namespace dpc {
+namespace {
+
+// Argument checks shared by both result-diff entry points (what: the caller's name).
+int check_result_diff_args(const uint64_t* d_old, const uint64_t* d_new, size_t n_bits,
+                           const uint32_t* d_changed_idx, const uint32_t* d_count, const char* what)
 {
-  if (!d_count || !d_workspace) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: null count/workspace");
-  if (n_bits == 0) return dpc_fill_u32(d_count, 0u, 1, stream);
+  if (!d_count) return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: null count", what);
+  if (n_bits == 0) return DPC_OK;
   if (!d_old || !d_new || !d_changed_idx || !aligned(d_old, 8) || !aligned(d_new, 8))
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: bad mask pointer");
-  if (n_bits > 0xFFFFFFFFull) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: more than 2^32 bits");
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: bad mask pointer", what);
+  if (n_bits > 0xFFFFFFFFull) return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: more than 2^32 bits", what);
+  return DPC_OK;
+}
+
+// The result diff's launches, with counts = one u32 per block of kDiffThreads
+// mask words (every one written by diff_count_kernel before it is read).
+int launch_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx,
+                       uint32_t* d_count, int update_old, uint32_t* counts, void* stream)
+{
+  if (n_bits == 0) return dpc_fill_u32(d_count, 0u, 1, stream);
   const size_t n_words = (n_bits + 63) / 64;
   const unsigned blocks = grid_for(n_words, kDiffThreads);
-  auto* counts = static_cast<uint32_t*>(d_workspace);
   auto* o = reinterpret_cast<unsigned long long*>(d_old);
   auto* nw = reinterpret_cast<const unsigned long long*>(d_new);
   hipLaunchKernelGGL(diff_count_kernel, dim3(blocks), dim3(kDiffThreads), 0, as_stream(stream), o, nw, n_words, n_bits,
@@ -765,6 +780,76 @@ int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint3
   hipLaunchKernelGGL(diff_write_kernel, dim3(blocks), dim3(kDiffThreads), 0, as_stream(stream), o, nw, n_words, n_bits,
                      counts, d_changed_idx, update_old);
   return check_launch("diff_write_kernel");
+}
+
+}  // namespace
+}  // namespace dpc
+
+// ---- allocation-free result diff ----------------------------------------------
+// The per-block counts of dpc_result_diff_noalloc live in this array, one copy
+// per device, which the library holds until the process ends (256 KiB: 65,536
+// blocks of 256 mask words, i.e. up to 2^30 bits).  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in): it looks up
+// each device's address once with hipGetSymbolAddress and passes that.
+static_assert(DPC_RESULT_DIFF_NOALLOC_MAX_BITS % (64ull * 256ull) == 0, "whole blocks of 256 mask words");
+static __device__ __align__(256) uint32_t scratch[DPC_RESULT_DIFF_NOALLOC_MAX_BITS / (64ull * 256ull)];
+
+namespace dpc {
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t* g_scratch_ptr[kScratchDevices] = {};  // per device, looked up on its first call
+
+// The current device's copy of the scratch array (no allocation: a fixed table
+// of per-device addresses, filled on each device's first call).
+int scratch_of_current_device(uint32_t** ptr, const char* what)
+{
+  int dev = 0;
+  DPC_HIP_TRY(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kScratchDevices)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: device %d is beyond the %d devices served", what, dev,
+                     kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return set_error(DPC_ERR_HIP, "%s: hipGetSymbolAddress returned no address", what);
+    g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return DPC_OK;
+}
+
+}  // namespace
+}  // namespace dpc
+
+extern "C" {
+
+int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx, uint32_t* d_count,
+                    int update_old, void* d_workspace, void* stream)
+{
+  if (!d_workspace) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_result_diff: null workspace");
+  const int rc = check_result_diff_args(d_old, d_new, n_bits, d_changed_idx, d_count, "dpc_result_diff");
+  if (rc) return rc;
+  return launch_result_diff(d_old, d_new, n_bits, d_changed_idx, d_count, update_old,
+                            static_cast<uint32_t*>(d_workspace), stream);
+}
+
+int dpc_result_diff_noalloc(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx,
+                            uint32_t* d_count, int update_old, void* stream)
+{
+  int rc = check_result_diff_args(d_old, d_new, n_bits, d_changed_idx, d_count, "dpc_result_diff_noalloc");
+  if (rc) return rc;
+  if (n_bits > DPC_RESULT_DIFF_NOALLOC_MAX_BITS)
+    return set_error(DPC_ERR_INVALID_ARGUMENT,
+                     "dpc_result_diff_noalloc: %zu bits exceed the %llu its fixed scratch serves (use dpc_result_diff)",
+                     n_bits, static_cast<unsigned long long>(DPC_RESULT_DIFF_NOALLOC_MAX_BITS));
+  uint32_t* counts = nullptr;
+  rc = scratch_of_current_device(&counts, "dpc_result_diff_noalloc");
+  if (rc) return rc;
+  return launch_result_diff(d_old, d_new, n_bits, d_changed_idx, d_count, update_old, counts, stream);
 }
 
 }  // extern "C"

@@ -48,6 +48,9 @@ def test_argument_errors_are_reported_before_any_device_call(lib):
     assert lib.dpc_cull(a, a, None, 10, a, 10, None, a, None) == inval
     # dpc_result_diff: no count output
     assert lib.dpc_result_diff(a, a, 100, a, None, 0, a, None) == inval
+    # dpc_result_diff_noalloc: more bits than its fixed scratch serves
+    assert lib.dpc_result_diff_noalloc(a, a, (1 << 30) + 64, a, a, 0, None) == inval
+    assert b"exceed" in lib.dpc_last_error()
     # dpc_inverse_transpose: more matrices than the leading dimension holds
     assert lib.dpc_inverse_transpose(a, 5, None, 10, a, 10, None, None) == inval
     assert b"exceeds" in lib.dpc_last_error()
