@@ -236,3 +236,139 @@ def test_serve_errors_and_scene_slots(orc):
         with pytest.raises(_lib.DpcError) as e:  # waited for already
             srv.wait(info.request_id, 10.0)
         assert e.value.code == _lib.DPC_ERR_NOT_FOUND
+
+
+def _submit_on_schedule(srv, ids, timeline, pick_masks):
+    """Submits timeline[i] at its arrival time on the server's clock (a spin
+    loop) while a second thread waits for the requests in order; returns every
+    request's result and masks (pick_masks(i, r) -> a uint64 array to receive them)."""
+    import sys
+    import threading
+
+    n = len(timeline)
+    ids_q = [None] * n
+    results = [None] * n
+    masks = [None] * n
+    cv = threading.Condition()
+
+    def waiter():
+        for i in range(n):
+            with cv:
+                while ids_q[i] is None:
+                    cv.wait()
+            results[i] = srv.wait(ids_q[i], 120_000.0)
+
+    switch = sys.getswitchinterval()
+    sys.setswitchinterval(2e-4)
+    th = threading.Thread(target=waiter)
+    th.start()
+    try:
+        start = srv.now_us() + 1000.0
+        for i, r in enumerate(timeline):
+            due = start + r.arrival_us
+            while srv.now_us() < due:
+                pass
+            masks[i] = pick_masks(i, r)
+            rid = srv.submit(ids[r.scene], r.vps, masks[i], r.delta_idx, r.delta_mats)
+            with cv:
+                ids_q[i] = rid
+                cv.notify_all()
+    finally:
+        th.join()
+        sys.setswitchinterval(switch)
+    return results, masks
+
+
+def test_serve_sixteen_resident_scenes_poisson(orc):
+    """The serving workload at its stated scale: 16 resident scenes of
+    log-uniform size in [10k, 2M] objects, Poisson arrivals at 5k and then 20k
+    requests/s (400 each), every request 1-32 frusta and a delta of 1-10 % of
+    its scene.  Deltas of three requests in four lie in pinned memory (read by
+    the device directly), the rest are copied.  Eight sampled requests per rate
+    are checked against the oracle's replay of their scene's deltas in
+    submission order, every frustum, bit for bit: the last request of the
+    largest scene, requests admitted while another batch was in flight, and
+    requests that shared their batch with other scenes."""
+    scs = S.serving_scenes(seed=4, n_scenes=16)
+    sizes = [sc.n for sc in scs]
+    assert len(scs) == 16 and min(sizes) >= 10_000 and max(sizes) <= 2_000_000 and max(sizes) > 1_000_000
+    pinned = []
+    for sc in scs:
+        pp = PinnedArray(sc.perm.shape, np.uint32)
+        pp.array[:] = sc.perm
+        pm = PinnedArray(sc.pool.shape, np.float32)
+        pm.array[:] = sc.pool
+        pinned += [pp, pm]
+    rate_runs = []
+    try:
+        with Server(max_scenes=16, max_batch_requests=16, streams=8, hbm_budget_bytes=32 << 30,
+                    batches_per_stream=2) as srv:
+            ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scs]
+            for seed, rate in ((41, 5000.0), (42, 20000.0)):
+                reqs = S.serving_requests(scs, seed=seed, n_requests=400, rate_per_s=rate)
+                timeline = []
+                for i, r in enumerate(reqs):
+                    if i % 4 != 3 and r.delta_idx.size:
+                        # the same rows, addressed inside the pinned copies of perm and pool
+                        sc = scs[r.scene]
+                        o1 = int((r.delta_idx.ctypes.data - sc.perm.ctypes.data) // 4)
+                        o2 = int((r.delta_mats.ctypes.data - sc.pool.ctypes.data) // 64)
+                        k = r.delta_idx.size
+                        pi, pm = pinned[2 * r.scene].array, pinned[2 * r.scene + 1].array
+                        timeline.append(S.ServingRequest(r.scene, r.vps, pi[o1:o1 + k], pm[o2:o2 + k], r.arrival_us))
+                    else:
+                        timeline.append(r)
+                results, masks = _submit_on_schedule(
+                    srv, ids, timeline, lambda i, r: np.zeros(srv.mask_words(ids[r.scene], r.vps.shape[0]), np.uint64))
+                # the replay below reads the original (pageable) views of the same rows:
+                # the pinned copies are freed before it runs
+                rate_runs.append((rate, reqs, results, masks))
+            stats = srv.stats()
+    finally:
+        for p in pinned:
+            p.free()
+    assert stats["delta_direct"] > 0
+    samples = []
+    for rate, timeline, results, masks in rate_runs:  # timeline: the requests in submission order
+        assert all(x.status == 0 for x in results), rate
+        assert max(x.batch_requests for x in results) > 1, f"{rate}/s: no batch held more than one request"
+        spans = sorted({(x.batch, x.admit_us, x.done_us) for x in results})
+        overlapped = [i for i, x in enumerate(results)
+                      if any(b != x.batch and a0 < x.admit_us < d0 for b, a0, d0 in spans)]
+        assert overlapped, f"{rate}/s: no request was admitted while another batch was in flight"
+        shared = [i for i, x in enumerate(results) if x.batch_requests > 1]
+        largest = int(np.argmax(sizes))
+        last_of_largest = max(i for i, r in enumerate(timeline) if r.scene == largest)
+        rng = np.random.default_rng(int(rate))
+        sample = {last_of_largest}
+        # then up to three admitted during another batch, two that shared a batch, the rest at random
+        for pool, upto in ((overlapped, 4), (shared, 6), (list(range(len(timeline))), 8)):
+            for i in rng.permutation(pool):
+                if len(sample) >= upto:
+                    break
+                sample.add(int(i))
+        assert len(sample) == 8
+        samples.append(sample)
+    # host replay of every delta of the sampled scenes in submission order, over
+    # both rates' requests (the second stream starts from the scenes the first left)
+    mats = {run[1][i].scene: scs[run[1][i].scene].matrices.reshape(-1, 16).copy()
+            for run, sample in zip(rate_runs, samples) for i in sample}
+    for (rate, timeline, results, masks), sample in zip(rate_runs, samples):
+        bad, checked = 0, 0
+        for i, r in enumerate(timeline):
+            if r.scene not in mats:
+                continue
+            mats[r.scene][r.delta_idx] = r.delta_mats.reshape(-1, 16)
+            if i not in sample:
+                continue
+            sc = scs[r.scene]
+            l4, e4 = S.extents(sc.lower, sc.upper)
+            obbs = orc.obb_build_group_mt(l4, e4, np.arange(sc.n), mats[r.scene], 16)
+            words = (sc.n + 63) // 64
+            for f in range(r.vps.shape[0]):
+                want = orc.cull_obbs_mt(obbs, r.vps[f], 16)
+                bad += mismatches(masks[i][f * words:(f + 1) * words], want, sc.n)
+                checked += sc.n
+        print(f"serving {rate:.0f}/s: {len(timeline)} requests, {len({x.batch for x in results})} batches, "
+              f"sampled {sorted(sample)} ({checked} tests), {bad} mismatching bits")
+        assert bad == 0, f"{rate}/s: {bad} visibility bits differ from the oracle"
