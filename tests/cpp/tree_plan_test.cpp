@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "dpcull/Tree.hpp"
@@ -275,8 +276,10 @@ static void grow(TreeLayout& t, size_t n, uint64_t seed, double p, uint32_t max_
   }
 }
 
-int main()
+int main(int argc, char** argv)
 {
+  // --small: everything but the 1M-node tree (the sanitizer build runs this)
+  const bool small = argc > 1 && std::string(argv[1]) == "--small";
   const Config real512{512, 1152, 64, 512}, real256{256, 576, 64, 1024}, real1024{1024, 2048, 64, 256};
   const Config tiny{32, 48, 8, 6}, tiny2{64, 96, 16, 3};
 
@@ -337,7 +340,7 @@ int main()
   }
 
   // the benchmark's shape: 1,000,000 nodes, geometric children (p = 0.4, at most 8), depth <= 16
-  {
+  if (!small) {
     TreeLayout t;
     grow(t, 1000000, 2, 0.4, 8, 16, 8, 1);
     const Layout lay = compile(t);

@@ -54,3 +54,19 @@ def test_xbar_integration_frames():
     the CPU manager restatement and a full cull of the current scene."""
     out = run("integration_test")
     assert "integration: 0 failures" in out
+
+
+def test_host_code_under_sanitizers():
+    """The tile planner (tree_plan.hpp) and the tree bookkeeping (TreeLayout in
+    host/Tree.cpp) built with AddressSanitizer and UndefinedBehaviorSanitizer
+    (g++, host code only) and run: random, index-reuse and adversarial trees,
+    and the reference's recorded Tree trace.  Any report fails the run."""
+    import os
+    subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp"), "sanitize"], check=True, timeout=600)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="halt_on_error=1")
+    for args in (["asan_tree_plan_test", "--small"], ["asan_tree_layout_test", str(GOLDEN / "ref_tree_trace.bin")]):
+        res = subprocess.run([str(BIN / args[0]), *args[1:]], capture_output=True, text=True, timeout=600,
+                             cwd=str(ROOT), env=env)
+        assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
+        assert "0 failures" in res.stdout
+        assert "runtime error" not in res.stderr and "AddressSanitizer" not in res.stderr, res.stderr[-4000:]
