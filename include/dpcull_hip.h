@@ -58,6 +58,12 @@ extern "C" {
  *      needs none, its GL sibling uses dp/gl buffers, opengl/src/GroupImpl.cpp:68-183) */
 int dpc_abi_version(void);
 const char* dpc_last_error(void);
+/* Test hook (no reference counterpart): after `after_calls` more checked HIP
+ * calls of this library succeed, the next one reports hip_error (a hipError_t
+ * value, as DPC_ERR_HIP) instead of its own result; after_calls < 0 turns the
+ * injection off.  Lets tests drive every error path of the host code (the C++
+ * mirrors turn it into std::runtime_error) and check the state afterwards. */
+int dpc_debug_inject_error(int64_t after_calls, int hip_error);
 int dpc_device_info(int device, char* name, size_t name_len, int* cu_count, int* clock_khz,
                     size_t* total_mem, int* l2_bytes);
 int dpc_set_device(int device);

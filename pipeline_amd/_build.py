@@ -76,7 +76,7 @@ def build(verbose: bool = False) -> Path:
     out = LIBDIR / LIBNAME
     tmp = LIBDIR / (LIBNAME + ".tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *objs,
-          f"-Wl,-rpath,{ROCM / 'lib'}", "-Wl,--no-undefined"])
+          f"-L{ROCM / 'lib'}", "-lrocprofiler-sdk-roctx", f"-Wl,-rpath,{ROCM / 'lib'}", "-Wl,--no-undefined"])
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")

@@ -80,6 +80,7 @@ FP = C.POINTER(C.c_float)
 _SIGNATURES = {
     "dpc_abi_version": (I, []),
     "dpc_last_error": (C.c_char_p, []),
+    "dpc_debug_inject_error": (I, [C.c_int64, I]),
     "dpc_device_info": (I, [I, C.c_char_p, SZ, C.POINTER(I), C.POINTER(I), C.POINTER(SZ), C.POINTER(I)]),
     "dpc_set_device": (I, [I]),
     "dpc_malloc": (I, [C.POINTER(P), SZ]),

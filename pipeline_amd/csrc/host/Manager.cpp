@@ -12,6 +12,8 @@
 
 #include "dpcull/Tree.hpp"
 #include "dpcull_hip.h"
+#include "trace.hpp"
+#include "unwind.hpp"
 
 namespace dpcull {
 namespace culling {
@@ -281,6 +283,7 @@ class ManagerImpl : public culling::Manager {
 
   void cull(GroupSharedPtr const& group, ResultSharedPtr const& result, Mat44f const& viewProjection) override
   {
+    TraceRange trace("cull");
     auto g = impl(group);
     auto r = impl(result);
     if (r->m_group != g) throw std::invalid_argument("cull: the result belongs to another group");
@@ -299,6 +302,7 @@ class ManagerImpl : public culling::Manager {
 
   Box3f getBoundingBox(GroupSharedPtr const& group) const override
   {
+    TraceRange trace("cull::getBoundingBox");
     auto g = const_cast<ManagerImpl*>(this)->impl(group);
     // a tree-bound group's world matrices change with every Tree::compute()
     if (g->m_tree && g->m_boundingBoxTreeCompute != g->m_tree->computeCount()) g->m_boundingBoxDirty = true;
@@ -378,6 +382,7 @@ class ManagerImpl : public culling::Manager {
     box_a.reserve(std::max<size_t>(4 * n, 4));
     box_b.reserve(std::max<size_t>(2 * n, 2));
     xform.reserve(std::max<size_t>(n, 1));
+    SyncOnUnwind sync(m_stream);  // (a, b, x are read by the copies)
     if (n) {
       check(dpc_memcpy_h2d(box_a.p, a.data(), a.size() * 4, m_stream), "upload boxes");
       check(dpc_memcpy_h2d(box_b.p, b.data(), b.size() * 4, m_stream), "upload boxes");
@@ -397,6 +402,7 @@ class ManagerImpl : public culling::Manager {
     std::vector<float> packed(16 * count);
     const char* base = static_cast<const char*>(g.m_matrices);
     for (size_t k = 0; k < count; ++k) std::memcpy(&packed[16 * k], base + k * g.m_matricesStride, 64);
+    SyncOnUnwind sync(m_stream);
     check(dpc_memcpy_h2d(g.aos.p, packed.data(), packed.size() * 4, m_stream), "upload matrices");
     check(dpc_matrices_from_aos(g.aos.p, 64, count, planes.p, count, m_stream), "matrices to planes");
     check(dpc_stream_sync(m_stream), "upload sync");
@@ -416,6 +422,7 @@ class ManagerImpl : public culling::Manager {
     if (idx.empty()) return;
     g.idx.reserve(idx.size());
     g.delta.reserve(mats.size());
+    SyncOnUnwind sync(m_stream);
     check(dpc_memcpy_h2d(g.idx.p, idx.data(), idx.size() * 4, m_stream), "upload delta index");
     check(dpc_memcpy_h2d(g.delta.p, mats.data(), mats.size() * 4, m_stream), "upload delta");
     check(dpc_matrices_scatter(g.planes.p, g.m_matricesCount, g.idx.p, g.delta.p, idx.size(), m_stream),
@@ -488,6 +495,7 @@ class ManagerImpl : public culling::Manager {
   // ResultBitSet::updateChanged (ResultBitSet.cpp:61-108) with the diff on the device.
   void updateResult(GroupImpl& g, ResultImpl& r, const uint64_t* d_new, size_t n)
   {
+    TraceRange trace("ResultBitSet::updateChanged");
     if (r.m_incarnation != g.m_incarnation) {
       r.m_incarnation = g.m_incarnation;
       const size_t old = r.m_size;
@@ -501,22 +509,35 @@ class ManagerImpl : public culling::Manager {
     r.d_count.reserve(1);
     const size_t ws = dpc_result_diff_workspace(n);
     r.d_ws.reserve(std::max<size_t>(ws, 4));
-    if (r.m_deviceStale && words) {
-      check(dpc_memcpy_h2d(r.d_old.p, r.m_bits.data(), words * 8, m_stream), "upload result bits");
-      r.m_deviceStale = false;
-    }
-    check(dpc_result_diff(r.d_old.p, d_new, n, r.d_changed.p, r.d_count.p, 1, r.d_ws.p, m_stream), "dpc_result_diff");
-    uint32_t count = 0;
-    check(dpc_memcpy_d2h(&count, r.d_count.p, 4, m_stream), "download count");
-    if (words) check(dpc_memcpy_d2h(r.m_bits.data(), d_new, words * 8, m_stream), "download result bits");
-    check(dpc_stream_sync(m_stream), "result sync");
-    std::vector<uint32_t> idx(count);
-    if (count) {
-      check(dpc_memcpy_d2h(idx.data(), r.d_changed.p, count * 4, m_stream), "download changed");
+    // The result changes only when every step succeeded: after a failure the
+    // host bits are still the last successful cull's, and the device copy,
+    // which the diff may already have overwritten, is uploaded again next time.
+    std::vector<uint64_t> bits(words);
+    std::vector<uint32_t> idx;
+    try {
+      if (r.m_deviceStale && words) {
+        check(dpc_memcpy_h2d(r.d_old.p, r.m_bits.data(), words * 8, m_stream), "upload result bits");
+        r.m_deviceStale = false;
+      }
+      check(dpc_result_diff(r.d_old.p, d_new, n, r.d_changed.p, r.d_count.p, 1, r.d_ws.p, m_stream),
+            "dpc_result_diff");
+      uint32_t count = 0;
+      check(dpc_memcpy_d2h(&count, r.d_count.p, 4, m_stream), "download count");
+      if (words) check(dpc_memcpy_d2h(bits.data(), d_new, words * 8, m_stream), "download result bits");
       check(dpc_stream_sync(m_stream), "result sync");
+      idx.resize(count);
+      if (count) {
+        check(dpc_memcpy_d2h(idx.data(), r.d_changed.p, count * 4, m_stream), "download changed");
+        check(dpc_stream_sync(m_stream), "result sync");
+      }
+    } catch (...) {
+      (void)dpc_stream_sync(m_stream);  // the host buffers above must outlive the copies already queued
+      r.m_deviceStale = true;
+      throw;
     }
+    std::copy(bits.begin(), bits.end(), r.m_bits.begin());
     r.m_changed.clear();
-    r.m_changed.reserve(count);
+    r.m_changed.reserve(idx.size());
     for (uint32_t i : idx) r.m_changed.push_back(g.m_objects[i]);
   }
 
@@ -574,6 +595,7 @@ class ManagerImpl : public culling::Manager {
 
 void ManagerImpl::cullBatch(std::vector<CullRequest> const& requests)
 {
+  TraceRange trace("cullBatch");
   if (requests.empty()) return;
   // one scene per distinct group; one pair per request
   std::vector<std::shared_ptr<GroupImpl>> groups;
@@ -621,6 +643,7 @@ void ManagerImpl::cullBatch(std::vector<CullRequest> const& requests)
   check(dpc_batch_plan_bind(plan, scenes.data(), scenes.size()), "dpc_batch_plan_bind");
   m_vps.reserve(vps.size());
   m_batchMasks.reserve(std::max<uint64_t>(words, 1));
+  SyncOnUnwind sync(m_stream);
   check(dpc_memcpy_h2d(m_vps.p, vps.data(), vps.size() * 4, m_stream), "upload frusta");
   check(dpc_cull_batch(plan, m_vps.p, m_batchMasks.p, std::max<uint64_t>(words, 1), m_stream), "dpc_cull_batch");
   // results in request order, exactly as the same sequence of cull() calls

@@ -30,6 +30,7 @@
 
 #include "../batch_types.hpp"
 #include "dpcull_hip.h"
+#include "trace.hpp"
 
 namespace dpc {
 int set_error(int code, const char* fmt, ...);  // runtime.hip: the thread's dpc_last_error() text
@@ -194,6 +195,7 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   snapshot.reserve(pick.size());
   for (auto& r : pick) snapshot.push_back(scenes[r->scene]);
   lock.unlock();
+  dpcull::TraceRange trace("serve batch issue");
   const double t_issue0 = now();
 
   // Arena layout: the batch descriptors (entries, pairs, frusta; staged in the

@@ -9,6 +9,8 @@
 #include <string>
 
 #include "dpcull_hip.h"
+#include "trace.hpp"
+#include "unwind.hpp"
 
 namespace dpcull {
 namespace transform {
@@ -328,10 +330,13 @@ void Tree::recompile()
 
 void Tree::compute(Mat44f const&)
 {
+  TraceRange trace("Update TransformTree");
   Device& d = *m_dev;
   const size_t cap = m_layout.capacity();
   std::vector<uint32_t> slots;  // host buffers stay alive until the stream is synchronised below
   std::vector<Mat44f> mats;
+  std::vector<uint8_t> dirty_by_slot;
+  SyncOnUnwind sync(d.stream);  // ... or, when a step fails, until the copies already queued are done
   if (m_structureDirty || d.ld != cap) {
     recompile();
   } else if (m_anyDirty) {
@@ -350,7 +355,7 @@ void Tree::compute(Mat44f const&)
       check(dpc_matrices_scatter(d.local.p, d.ld, d.idx.p, d.delta.p, slots.size(), d.stream), "scatter delta");
     }
   }
-  std::vector<uint8_t> dirty_by_slot(cap, 0);
+  dirty_by_slot.assign(cap, 0);
   for (size_t s = 0; s < cap; ++s) dirty_by_slot[s] = m_dirtyLocal[m_compiled.index_of_slot[s]];
   check(dpc_memcpy_h2d(d.dirty_local.p, dirty_by_slot.data(), cap, d.stream), "upload dirty");
   check(dpc_tree_propagate_plan(d.plan, d.local.p, d.world.p, d.dirty_local.p, d.dirty_world.p, d.stream),

@@ -22,10 +22,15 @@ inline int hip_fail(hipError_t e, const char* what)
 
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
+// Error injection for tests (dpc_debug_inject_error): the checked HIP call
+// that brings the countdown to zero reports the injected error instead of its
+// own result.  Off (a negative countdown) unless a test arms it.
+hipError_t injected(hipError_t e);
+
 // Launch check after a kernel launch: reports launch-configuration errors.
 inline int check_launch(const char* what)
 {
-  hipError_t e = hipGetLastError();
+  hipError_t e = injected(hipGetLastError());
   if (e != hipSuccess) return hip_fail(e, what);
   return DPC_OK;
 }
@@ -41,6 +46,6 @@ constexpr int kWave = 64;
 
 #define DPC_HIP_TRY(call, what)                       \
   do {                                                \
-    hipError_t e_ = (call);                           \
+    hipError_t e_ = dpc::injected(call);              \
     if (e_ != hipSuccess) return dpc::hip_fail(e_, what); \
   } while (0)

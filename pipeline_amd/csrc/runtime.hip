@@ -1,5 +1,6 @@
 // runtime.hip -- device memory, streams, events, graphs and small utility
 // kernels behind the C ABI.
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -8,6 +9,8 @@
 namespace dpc {
 
 namespace {
+std::atomic<long long> g_inject_countdown{-1};  // < 0: off
+std::atomic<int> g_inject_error{hipSuccess};
 thread_local std::string g_last_error;
 std::mutex g_cu_mutex;
 std::vector<int> g_cu_count;  // per device, 0 = unknown
@@ -22,6 +25,14 @@ int set_error(int code, const char* fmt, ...)
   va_end(ap);
   g_last_error = buf;
   return code;
+}
+
+hipError_t injected(hipError_t e)
+{
+  if (g_inject_countdown.load(std::memory_order_relaxed) < 0) return e;
+  if (g_inject_countdown.fetch_sub(1, std::memory_order_relaxed) == 0)
+    return static_cast<hipError_t>(g_inject_error.load(std::memory_order_relaxed));
+  return e;
 }
 
 int device_cu_count()
@@ -54,6 +65,15 @@ extern "C" {
 
 int dpc_abi_version(void) { return DPC_ABI_VERSION; }
 
+int dpc_debug_inject_error(int64_t after_calls, int hip_error)
+{
+  if (after_calls >= 0 && hip_error == 0)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_debug_inject_error: the injected error must be nonzero");
+  g_inject_error.store(hip_error, std::memory_order_relaxed);
+  g_inject_countdown.store(after_calls < 0 ? -1 : after_calls, std::memory_order_relaxed);
+  return DPC_OK;
+}
+
 const char* dpc_last_error(void) { return g_last_error.c_str(); }
 
 int dpc_device_info(int device, char* name, size_t name_len, int* cu_count, int* clock_khz, size_t* total_mem,
@@ -82,7 +102,11 @@ int dpc_malloc(void** d_ptr, size_t bytes)
   if (!d_ptr) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_malloc: null output pointer");
   *d_ptr = nullptr;
   if (bytes == 0) return DPC_OK;
-  hipError_t e = hipMalloc(d_ptr, bytes);
+  hipError_t e = injected(hipMalloc(d_ptr, bytes));
+  if (e != hipSuccess && *d_ptr) {  // an injected failure of a call that succeeded
+    (void)hipFree(*d_ptr);
+    *d_ptr = nullptr;
+  }
   if (e == hipErrorOutOfMemory) return set_error(DPC_ERR_OUT_OF_MEMORY, "dpc_malloc: out of memory (%zu bytes)", bytes);
   if (e != hipSuccess) return hip_fail(e, "hipMalloc");
   return DPC_OK;

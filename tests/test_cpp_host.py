@@ -70,3 +70,13 @@ def test_host_code_under_sanitizers():
         assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
         assert "0 failures" in res.stdout
         assert "runtime error" not in res.stderr and "AddressSanitizer" not in res.stderr, res.stderr[-4000:]
+
+
+@pytest.mark.gpu
+def test_host_mirrors_recover_from_failing_hip_calls():
+    """dpc_debug_inject_error makes the k-th checked HIP call fail: for every k
+    inside a Tree::compute with a relayout and inside a Manager::cull, the call
+    throws, and the retry gives the results of a run that never failed (world
+    matrices vs the oracle; changed lists and visibility vs the CPU manager)."""
+    out = run("fault_test")
+    assert "fault injection: 0 failures" in out
