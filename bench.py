@@ -48,6 +48,7 @@ CMPS_PER_TEST = 48               # 8 corners x 6 clip compares
 OPS_PER_OBJECT = FLOPS_PER_OBJECT
 OPS_PER_TEST = FLOPS_PER_TEST + CMPS_PER_TEST  # VALU operations, compares included
 BYTES_PER_OBJECT = 88            # 64 B matrix + 24 B box
+GRAPH_STEPS = 10                 # steps captured in one graph for the graph-of-steps timing
 
 
 def parse():
@@ -56,6 +57,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--skip-extra", action="store_true", help="only the batched and per-view lines")
+    ap.add_argument("--skip-serving", action="store_true", help="every line but the serving measurement")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-probe", action="store_true",
                     help="launch only a 512 MiB device copy, the batched kernel, the flat kernel, the tree "
@@ -119,8 +121,8 @@ def run_timed(dev, lib, stream, steps, warmup, issue):
 def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
     """The steps beside the path, on the tree workload's data: the group bounding
     box of its 2.37M objects, the result diff of its cull mask, the inverse
-    transpose of its 1M world matrices, and propagation with 1 % of the local
-    matrices dirty.  Kernel time per call from HIP events over 50 calls; their
+    transpose of its 1M world matrices, and propagation with 1 % and with all of
+    the local matrices dirty.  Kernel time per call from HIP events over 50 calls; their
     parity with the oracle is what tests/test_gpu_reduce.py, test_gpu_invert.py
     and test_gpu_tree.py check."""
     from pipeline_amd import _lib
@@ -129,7 +131,7 @@ def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
     n_slots = ts.n_slots
     m = objs.n_objects
     # group bounding box (ManagerBitSet::calculateBoundingBox)
-    ws = dev.DeviceArray(max(lib.dpc_bbox_workspace(m), 4), np.uint8)
+    ws = dev.workspace(lib.dpc_bbox_workspace(m))
     box6 = dev.DeviceArray(8, np.float32)
 
     def issue_bbox():
@@ -152,7 +154,7 @@ def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
     d_old = dev.DeviceArray.from_host(old_h)
     d_idx = dev.DeviceArray(max(flip.size, 1), np.uint32)
     d_cnt = dev.DeviceArray(1, np.uint32)
-    ws = dev.DeviceArray(max(lib.dpc_result_diff_workspace(m), 4), np.uint8)
+    ws = dev.workspace(lib.dpc_result_diff_workspace(m))
 
     def issue_diff():
         _lib.check(lib.dpc_result_diff(d_old.ptr, tmask.ptr, m, d_idx.ptr, d_cnt.ptr, 0, ws.ptr, stream.handle))
@@ -191,13 +193,19 @@ def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
     changed = int(d_dw.download(stream.handle)[1:n_slots].astype(np.int64).sum())
     out["dirty_propagation"] = dict(nodes=n_slots - 1, dirty_local=int(dl.sum()), changed_world=changed,
                                     us=k_ms * 1e3)
+    # the same with every local matrix dirty (what a first compute after a relayout sees)
+    dl[1:n_slots] = 1
+    d_dl.upload(dl)
+    _, k_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_dirty)
+    out["dirty_propagation_all"] = dict(nodes=n_slots - 1, us=k_ms * 1e3)
     d_dl.free()
     d_dw.free()
     log("next rows: bbox {:.1f} us ({:.0f} GB/s), result diff {:.1f} us ({} of {} changed), inverse transpose "
         "{:.1f} us ({:.0f} GB/s), propagation with 1 % dirty {:.1f} us ({} world matrices changed)".format(
             out["bbox"]["us"], out["bbox"]["gbs"], out["result_diff"]["us"], out["result_diff"]["changed"],
             out["result_diff"]["bits"], out["inverse_transpose"]["us"], out["inverse_transpose"]["gbs"],
-            out["dirty_propagation"]["us"], out["dirty_propagation"]["changed_world"]))
+            out["dirty_propagation"]["us"], out["dirty_propagation"]["changed_world"])
+        + ", with every local dirty {:.1f} us".format(out["dirty_propagation_all"]["us"]))
     return out
 
 
@@ -481,7 +489,7 @@ def pmc_probe():
     stream.sync()
     # group bounding box of the tree's 2,372,872 objects over the propagated world matrices
     objs = dev.upload_scene(ts.obj_lower, ts.obj_upper, np.eye(4, dtype=np.float32)[None], ts.obj_slot)
-    ws = dev.DeviceArray(max(lib.dpc_bbox_workspace(objs.n_objects), 8), np.uint8)
+    ws = dev.workspace(lib.dpc_bbox_workspace(objs.n_objects))
     box6 = dev.DeviceArray(8, np.float32)
     for _ in range(5):
         _lib.check(lib.dpc_bbox(objs.box_a.ptr, objs.box_b.ptr, objs.xform_addr, objs.n_objects, d_world.ptr,
@@ -559,6 +567,12 @@ def main():
     # this runtime a replay costs more per step than a direct launch)
     graph = dev.Graph(stream, issue_batch)
     wall_g, _, _ = run_timed(dev, lib, stream, args.steps, args.warmup, lambda: graph.launch(stream))
+    # a graph holding GRAPH_STEPS steps: the runtime's cost per replay (about 5 us
+    # on this stack) is paid once per GRAPH_STEPS steps
+    graph_k = dev.Graph(stream, lambda: [issue_batch() for _ in range(GRAPH_STEPS)])
+    reps = max(args.steps // GRAPH_STEPS, 1)
+    wall_gk, _, _ = run_timed(dev, lib, stream, reps, max(args.warmup // GRAPH_STEPS, 1), lambda: graph_k.launch(stream))
+    graph_k.destroy()
     wall_v, k_mean_v, k_med_v = run_timed(dev, lib, stream, args.steps, args.warmup, issue_per_view)
     batched = masks.download(stream.handle)
     per_view = masks_pv.download(stream.handle)
@@ -588,11 +602,14 @@ def main():
     flops = FLOPS_PER_OBJECT * referenced + FLOPS_PER_TEST * tests
     achieved_tflops = flops / (k_mean_b * 1e-3) / 1e12
     log(f"batched: {k_mean_b * 1e3:.1f} us/launch (event), {wall_b / args.steps * 1e6:.1f} us/step wall "
-        f"({wall_g / args.steps * 1e6:.1f} us/step as graph replays), "
+        f"({wall_g / args.steps * 1e6:.1f} us/step as graph replays, "
+        f"{wall_gk / (reps * GRAPH_STEPS) * 1e6:.1f} us/step in graphs of {GRAPH_STEPS} steps), "
         f"{value / 1e9:.2f} G tests/s; per-view {wall_v / args.steps * 1e6:.1f} us/step, "
         f"{per_view_value / 1e9:.2f} G tests/s; mismatches vs per-view {mism_pv}, vs oracle sample {mism_orc}")
     result["batched"] = dict(value=value, ms_per_step=wall_b / args.steps * 1e3, kernel_us=k_mean_b * 1e3,
                              graph_replay_us_per_step=wall_g / args.steps * 1e6,
+                             graph_of_steps=GRAPH_STEPS,
+                             graph_of_steps_us_per_step=wall_gk / (reps * GRAPH_STEPS) * 1e6,
                              kernel_us_median=k_med_b * 1e3, tests=tests, objects_referenced=referenced,
                              algorithmic_ops=ops, algorithmic_bytes=hbm_bytes,
                              algorithmic_flops=flops, achieved_tflops=achieved_tflops,
@@ -721,6 +738,10 @@ def main():
         tgraph = dev.Graph(stream, issue_tree)
         wall_tg, _, _ = run_timed(dev, lib, stream, t_steps, args.warmup, lambda: tgraph.launch(stream))
         tgraph.destroy()
+        tgraph_k = dev.Graph(stream, lambda: [issue_tree() for _ in range(GRAPH_STEPS)])
+        t_reps = max(t_steps // GRAPH_STEPS, 1)
+        wall_tgk, _, _ = run_timed(dev, lib, stream, t_reps, 1, lambda: tgraph_k.launch(stream))
+        tgraph_k.destroy()
         # parity of this run's results: world matrices vs the FP32 oracle (ulp) and
         # an FP64 evaluation of the same recurrence; the cull mask vs the oracle
         got_w = np.transpose(d_world.download(stream.handle), (1, 0, 2))
@@ -751,13 +772,15 @@ def main():
         result["tree"] = dict(nodes=n_nodes, levels=len(ts.level_offsets) - 1, objects=m_obj,
                               us_per_step=wall_t / t_steps * 1e6, step_event_us=k_mean_t * 1e3,
                               graph_replay_us_per_step=wall_tg / t_steps * 1e6,
+                              graph_of_steps_us_per_step=wall_tgk / (t_reps * GRAPH_STEPS) * 1e6,
                               achieved_gbs=(prop_bytes + cull_bytes) / (k_mean_t * 1e-3) / 1e9,
                               tests_per_s=m_obj / (wall_t / t_steps),
                               world_max_ulp_vs_oracle=max_ulp, world_max_rel_err_vs_fp64=max_rel,
                               cull_mismatches=mism_t,
                               propagation_launches=t_launches.value, tile_runs=t_runs.value, tiles=t_tiles.value)
         log(f"tree: {result['tree']['us_per_step']:.1f} us/step (propagate + cull; "
-            f"{result['tree']['graph_replay_us_per_step']:.1f} us as graph replays)")
+            f"{result['tree']['graph_replay_us_per_step']:.1f} us as graph replays, "
+            f"{result['tree']['graph_of_steps_us_per_step']:.1f} us in graphs of {GRAPH_STEPS} steps)")
         result["next_rows"] = bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask)
         for d in (d_local, d_world, d_dirty, tmask):
             d.free()
@@ -765,7 +788,8 @@ def main():
         lib.dpc_tree_plan_destroy(tplan)
 
         # ---------------- serving loop ----------------
-        result["serving"] = bench_serving(args, lib, orc)
+        if not args.skip_serving:
+            result["serving"] = bench_serving(args, lib, orc)
 
     stream.destroy()
     line = {

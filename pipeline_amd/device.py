@@ -65,6 +65,16 @@ class DeviceArray:
             pass
 
 
+def workspace(nbytes: int, stream=None) -> DeviceArray:
+    """A device workspace for dpc_result_diff / dpc_bbox: zero before its first
+    use (the calls keep their state in it and leave it consistent)."""
+    ws = DeviceArray(max((int(nbytes) + 7) // 8, 1), np.uint64)
+    ws.fill_u32(0, stream)
+    if stream is None:
+        check(_lib.load().dpc_stream_sync(None))
+    return ws
+
+
 class Stream:
     def __init__(self):
         self.handle = C.c_void_p()

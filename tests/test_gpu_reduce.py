@@ -17,7 +17,7 @@ def gpu_bbox(lower, upper, mats, xform=None):
     lib = _lib.load()
     scene = dev.upload_scene(lower, upper, mats, xform)
     n = lower.shape[0]
-    ws = dev.DeviceArray(max(lib.dpc_bbox_workspace(n), 4), np.uint8)
+    ws = dev.workspace(lib.dpc_bbox_workspace(n))
     out = dev.DeviceArray(6, F)
     _lib.check(lib.dpc_bbox(scene.box_a.ptr, scene.box_b.ptr, scene.xform_addr, n, scene.planes.ptr, scene.ld,
                             out.ptr, ws.ptr, None))
@@ -43,7 +43,7 @@ def test_bbox_empty_and_signed_zero(orc):
     # an empty group returns (-FLT_MAX, +FLT_MAX) like the reference's Box3f(min, max)
     fmax = np.finfo(F).max
     lib = _lib.load()
-    ws = dev.DeviceArray(max(lib.dpc_bbox_workspace(0), 4), np.uint8)
+    ws = dev.workspace(lib.dpc_bbox_workspace(0))
     out = dev.DeviceArray(6, F)
     dummy = dev.DeviceArray(64, F)
     _lib.check(lib.dpc_bbox(dummy.ptr, dummy.ptr, None, 0, dummy.ptr, 1, out.ptr, ws.ptr, None))
@@ -118,7 +118,7 @@ def test_result_diff_matches_oracle(orc, n):
     d_new = dev.DeviceArray.from_host(new)
     d_idx = dev.DeviceArray(n, np.uint32)
     d_cnt = dev.DeviceArray(1, np.uint32)
-    ws = dev.DeviceArray(lib.dpc_result_diff_workspace(n), np.uint8)
+    ws = dev.workspace(lib.dpc_result_diff_workspace(n))
     _lib.check(lib.dpc_result_diff(d_old.ptr, d_new.ptr, n, d_idx.ptr, d_cnt.ptr, 1, ws.ptr, None))
     cnt = int(d_cnt.download()[0])
     want = orc.result_diff(n, old, new)
@@ -126,4 +126,116 @@ def test_result_diff_matches_oracle(orc, n):
     assert np.array_equal(d_idx.download()[:cnt], want)
     assert np.array_equal(d_old.download(), new)   # old = new afterwards
     for d in (d_old, d_new, d_idx, d_cnt, ws):
+        d.free()
+
+
+def _diff_inputs(n, seed, rate=0.01):
+    rng = np.random.default_rng(seed)
+    words = (n + 63) // 64
+    old = rng.integers(0, 2 ** 63, words, dtype=np.uint64) * np.uint64(2) + rng.integers(0, 2, words, dtype=np.uint64)
+    new = old ^ np.packbits(rng.random(words * 64) < rate, bitorder="little").view(np.uint64)
+    if n % 64:
+        keep = np.uint64((1 << (n % 64)) - 1)
+        old[-1] &= keep
+        new[-1] &= keep
+    return old, new
+
+
+def test_result_diff_one_workspace_many_calls(orc):
+    """One workspace for a sequence of calls of different sizes (different
+    workgroup counts): the look-back state each call leaves serves the next
+    one, with no clearing in between."""
+    lib = _lib.load()
+    sizes = [20_000_000, 1_000, 2_000_000, 65, 20_000_000, 16_385, 4_000_000, 1]
+    nmax = max(sizes)
+    ws = dev.workspace(lib.dpc_result_diff_workspace(nmax))
+    d_idx = dev.DeviceArray(nmax, np.uint32)
+    d_cnt = dev.DeviceArray(1, np.uint32)
+    for k, n in enumerate(sizes):
+        old, new = _diff_inputs(n, 100 + k, 0.002 if n > 10_000_000 else 0.05)
+        d_old = dev.DeviceArray.from_host(old)
+        d_new = dev.DeviceArray.from_host(new)
+        _lib.check(lib.dpc_result_diff(d_old.ptr, d_new.ptr, n, d_idx.ptr, d_cnt.ptr, k % 2, ws.ptr, None))
+        cnt = int(d_cnt.download()[0])
+        want = orc.result_diff(n, old, new)
+        assert cnt == want.size, (k, n, cnt, want.size)
+        assert np.array_equal(d_idx.download()[:cnt], want), (k, n)
+        assert np.array_equal(d_old.download(), new if k % 2 else old), (k, n)
+        d_old.free()
+        d_new.free()
+    for d in (ws, d_idx, d_cnt):
+        d.free()
+
+
+def test_result_diff_graph_replays(orc):
+    """The diff captured once in a hipGraph and replayed with new masks written
+    into its inputs between replays: every replay equals the oracle (the look-back
+    tag comes from device memory, not from the captured arguments)."""
+    lib = _lib.load()
+    n = 3_000_000
+    stream = dev.Stream()
+    old0, new0 = _diff_inputs(n, 7)
+    d_old = dev.DeviceArray.from_host(old0)
+    d_new = dev.DeviceArray.from_host(new0)
+    d_idx = dev.DeviceArray(n, np.uint32)
+    d_cnt = dev.DeviceArray(1, np.uint32)
+    ws = dev.workspace(lib.dpc_result_diff_workspace(n), stream.handle)
+    stream.sync()
+    graph = dev.Graph(stream, lambda: _lib.check(lib.dpc_result_diff(d_old.ptr, d_new.ptr, n, d_idx.ptr, d_cnt.ptr, 0,
+                                                                     ws.ptr, stream.handle)))
+    for rep in range(5):
+        old, new = _diff_inputs(n, 200 + rep, 0.001 * (rep + 1))
+        d_old.upload(old, stream.handle)
+        d_new.upload(new, stream.handle)
+        graph.launch(stream)
+        cnt = int(d_cnt.download(stream.handle)[0])
+        want = orc.result_diff(n, old, new)
+        assert cnt == want.size, (rep, cnt, want.size)
+        assert np.array_equal(d_idx.download(stream.handle)[:cnt], want), rep
+    graph.destroy()
+    for d in (d_old, d_new, d_idx, d_cnt, ws):
+        d.free()
+    stream.destroy()
+
+
+def test_bbox_one_workspace_many_calls_and_graph(orc):
+    """One bounding-box workspace for calls over different object counts (so
+    different workgroup counts), then a captured call replayed after the
+    matrices move: every box equals the oracle's."""
+    lib = _lib.load()
+    rng = np.random.default_rng(12)
+    sizes = [2_000_000, 70, 300_000, 1, 2_000_000, 5_000]
+    ws = dev.workspace(max(lib.dpc_bbox_workspace(n) for n in sizes))
+    out = dev.DeviceArray(6, F)
+    for n in sizes:
+        lower, upper = scenes.random_boxes(rng, n, (-1000.0, 1000.0))
+        mats = scenes.trs_matrices(rng, n)
+        scene = dev.upload_scene(lower, upper, mats)
+        _lib.check(lib.dpc_bbox(scene.box_a.ptr, scene.box_b.ptr, scene.xform_addr, n, scene.planes.ptr, scene.ld,
+                                out.ptr, ws.ptr, None))
+        l4, e4 = scenes.extents(lower, upper)
+        want = orc.bbox_group(l4, e4, np.arange(n), mats)
+        assert np.array_equal(out.download().view(np.uint32), want.view(np.uint32)), n
+        scene.free()
+    n = 500_000
+    lower, upper = scenes.random_boxes(rng, n, (-1000.0, 1000.0))
+    mats = scenes.trs_matrices(rng, n)
+    scene = dev.upload_scene(lower, upper, mats)
+    stream = dev.Stream()
+    graph = dev.Graph(stream, lambda: _lib.check(lib.dpc_bbox(scene.box_a.ptr, scene.box_b.ptr, scene.xform_addr, n,
+                                                              scene.planes.ptr, scene.ld, out.ptr, ws.ptr,
+                                                              stream.handle)))
+    l4, e4 = scenes.extents(lower, upper)
+    for rep in range(4):
+        mats = scenes.trs_matrices(rng, n) * np.float32(1 + rep)
+        planes = np.ascontiguousarray(np.transpose(mats, (1, 0, 2))).reshape(-1)  # row planes
+        scene.planes.upload(planes, stream.handle)
+        graph.launch(stream)
+        got = out.download(stream.handle)
+        want = orc.bbox_group(l4, e4, np.arange(n), mats)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), rep
+    graph.destroy()
+    stream.destroy()
+    scene.free()
+    for d in (ws, out):
         d.free()

@@ -37,7 +37,7 @@ def run_diff(n, old, new, noalloc):
         if noalloc:
             _lib.check(lib.dpc_result_diff_noalloc(d_old.ptr, d_new.ptr, n, d_idx.ptr, d_cnt.ptr, 1, None))
         else:
-            ws = dev.DeviceArray(lib.dpc_result_diff_workspace(n), np.uint8)
+            ws = dev.workspace(lib.dpc_result_diff_workspace(n))
             _lib.check(lib.dpc_result_diff(d_old.ptr, d_new.ptr, n, d_idx.ptr, d_cnt.ptr, 1, ws.ptr, None))
         cnt = int(d_cnt.download()[0])
         return cnt, d_idx.download()[:cnt].copy(), d_old.download()
