@@ -834,7 +834,8 @@ def main():
                              "basis": f"{BYTES_PER_OBJECT} B per referenced object + 1 bit per test"}},
         "cpu_baseline": {"value": cpu_value, "unit": "tests/s", "cores": 1, "kind": "port",
                          "sample": f"{cpu_tests} tests: {pairs_done} (frustum, group) pairs over {passes} pass(es) of this "
-                                   "workload, OBBs built once per group and pass, then the per-object test loop"},
+                                   "workload, OBBs built once per group and pass, then the per-object test loop; "
+                                   "gcc -O2 -ffp-contract=off for the x86-64 baseline ISA (no AVX-512, no FMA)"},
         "launch_shapes": tuned,
         "device": info,
         "host": {"cpu": cpu_model(), "nproc": os.cpu_count(),
