@@ -156,6 +156,7 @@ class Tree {
   std::vector<TreeObserver*> m_observers;
   std::unique_ptr<Device> m_dev;
   uint64_t m_computeCount = 0;
+  bool m_dirtyWorldStale = false;  // a failed compute() may have left dirty-world bytes set
 };
 
 }  // namespace transform

@@ -576,6 +576,7 @@ class ManagerImpl : public culling::Manager {
     }
     g.bbox_ws.reserve(std::max<size_t>(dpc_bbox_workspace(n), 4));
     g.bbox_out.reserve(6);
+    SyncOnUnwind sync(m_stream);  // (the launch reads a, b, x and planes, freed on the way out)
     check(dpc_bbox(pa, pb, px, n, pp, ld, g.bbox_out.p, g.bbox_ws.p, m_stream), "dpc_bbox");
     float out[6];
     check(dpc_memcpy_d2h(out, g.bbox_out.p, sizeof out, m_stream), "download bbox");

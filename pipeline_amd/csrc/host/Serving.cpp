@@ -355,7 +355,15 @@ void dpc_server::run()
     bool any_busy = false;
     for (auto& slot : slots) {
       if (slot.busy) {
-        if (hipEventQuery(slot.done) == hipSuccess) {
+        const hipError_t q = hipEventQuery(slot.done);
+        if (q == hipSuccess) {
+          complete(slot);
+          progressed = true;
+        } else if (q != hipErrorNotReady) {
+          // the batch cannot be observed to finish (the device reported an
+          // error): its requests fail instead of waiting for ever
+          (void)hipGetLastError();
+          for (auto& r : slot.reqs) r->status = DPC_ERR_HIP;
           complete(slot);
           progressed = true;
         } else {
@@ -590,7 +598,9 @@ int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32
     }
   }
   std::lock_guard<std::mutex> g(s->mu);
-  if (scene_id >= s->scenes.size() || !s->scenes[scene_id].live) return fail(DPC_ERR_NOT_FOUND, "no such scene");
+  // (a scene still being uploaded has no id for its caller yet, and no sizes)
+  if (scene_id >= s->scenes.size() || !s->scenes[scene_id].live || !s->scenes[scene_id].ready)
+    return fail(DPC_ERR_NOT_FOUND, "no such scene");
   const Scene& sc = s->scenes[scene_id];
   if (sc.n_mat > dpc::kMaxHostScatterLd) {  // the direct scatter addresses the planes with 32-bit offsets
     r->didx_dev = nullptr;

@@ -337,22 +337,40 @@ void Tree::compute(Mat44f const&)
   std::vector<Mat44f> mats;
   std::vector<uint8_t> dirty_by_slot;
   SyncOnUnwind sync(d.stream);  // ... or, when a step fails, until the copies already queued are done
-  if (m_structureDirty || d.ld != cap) {
-    recompile();
-  } else if (m_anyDirty) {
-    // upload only the changed local matrices
-    for (size_t i = 0; i < cap; ++i) {
-      if (m_dirtyLocal[i]) {
-        slots.push_back(m_compiled.slot_of_index[i]);
-        mats.push_back(m_local[i]);
-      }
+  // A failure after the propagation was queued leaves its dirty-world bytes
+  // set; they are cleared before the next propagation, so that its event holds
+  // only the matrices it changes.
+  struct StaleOnUnwind {
+    bool& stale;
+    int exceptions = std::uncaught_exceptions();
+    ~StaleOnUnwind()
+    {
+      if (std::uncaught_exceptions() > exceptions) stale = true;
     }
-    if (!slots.empty()) {
-      d.idx.reserve(slots.size());
-      d.delta.reserve(16 * slots.size());
-      check(dpc_memcpy_h2d(d.idx.p, slots.data(), slots.size() * 4, d.stream), "upload delta index");
-      check(dpc_memcpy_h2d(d.delta.p, mats.data(), mats.size() * 64, d.stream), "upload delta");
-      check(dpc_matrices_scatter(d.local.p, d.ld, d.idx.p, d.delta.p, slots.size(), d.stream), "scatter delta");
+  } mark_stale{m_dirtyWorldStale};
+  if (m_structureDirty || d.ld != cap) {
+    recompile();  // (every local uploaded; new dirty-world bytes, cleared)
+    m_dirtyWorldStale = false;
+  } else {
+    if (m_dirtyWorldStale) {
+      check(dpc_fill_u32(reinterpret_cast<uint32_t*>(d.dirty_world.p), 0u, (cap + 4) / 4, d.stream), "clear dirty");
+      m_dirtyWorldStale = false;
+    }
+    if (m_anyDirty) {
+      // upload only the changed local matrices
+      for (size_t i = 0; i < cap; ++i) {
+        if (m_dirtyLocal[i]) {
+          slots.push_back(m_compiled.slot_of_index[i]);
+          mats.push_back(m_local[i]);
+        }
+      }
+      if (!slots.empty()) {
+        d.idx.reserve(slots.size());
+        d.delta.reserve(16 * slots.size());
+        check(dpc_memcpy_h2d(d.idx.p, slots.data(), slots.size() * 4, d.stream), "upload delta index");
+        check(dpc_memcpy_h2d(d.delta.p, mats.data(), mats.size() * 64, d.stream), "upload delta");
+        check(dpc_matrices_scatter(d.local.p, d.ld, d.idx.p, d.delta.p, slots.size(), d.stream), "scatter delta");
+      }
     }
   }
   dirty_by_slot.assign(cap, 0);

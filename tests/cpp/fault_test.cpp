@@ -7,6 +7,8 @@
 // (and so must an operation whose failing call was one whose status it ignores):
 //   Tree::compute after structural changes (the relayout, the plan, the
 //     propagation, the event): world matrices equal the oracle's;
+//   Tree::compute with moved transforms only: world matrices, and the event
+//     holds exactly the moved transforms and their descendants;
 //   Manager::cull with changed matrices and objects (uploads, the cull, the
 //     result diff): changed lists and visibility equal the CPU manager's,
 //     which ran without failures.
@@ -153,6 +155,56 @@ static void tree_compute_failures()
   std::printf("  tree: %d of %d computes failed (call k failing, k = 0..%d) and recovered\n", threw, k, k - 1);
 }
 
+// The same without structural changes (no relayout): a compute that fails
+// after its propagation was queued leaves dirty-world bytes behind, and the
+// retry's event must still hold exactly the matrices the retry changes: the
+// moved transforms and their descendants.
+static void tree_event_after_failures()
+{
+  std::mt19937 rng(5);
+  transform::Tree tree;
+  std::vector<Mat44f> local(1 << 17, Mat44f{});
+  local[0] = Mat44f::identity();
+  std::vector<transform::Index> nodes, parent_of(1 << 17, 0);
+  for (int k = 0; k < 4000; ++k) {
+    const transform::Index parent = nodes.empty() || rng() % 8 == 0 ? 0 : nodes[rng() % nodes.size()];
+    const Mat44f m = rand_local(rng);
+    const transform::Index i = tree.addTransform(parent, m);
+    local[i] = m;
+    parent_of[i] = parent;
+    nodes.push_back(i);
+  }
+  tree.compute();
+  int threw = 0, k = 0;
+  for (; k < 24; ++k) {
+    std::vector<uint8_t> moved(1 << 17, 0);
+    for (int j = 0; j < 30; ++j) {
+      const transform::Index i = nodes[rng() % nodes.size()];
+      local[i] = rand_local(rng);
+      tree.updateLocalMatrix(i, local[i]);
+      moved[i] = 1;
+    }
+    if (fails_at(k, [&] { tree.compute(); })) {
+      ++threw;
+      tree.compute();
+    }
+    // nodes are listed parents first: a node changes when it moved or its parent changed
+    std::vector<uint8_t> changed(1 << 17, 0);
+    for (transform::Index i : nodes) changed[i] = moved[i] | changed[parent_of[i]];
+    const auto& ev = tree.getLastChangedWorldMatrices();
+    size_t wrong = 0;
+    for (transform::Index i : nodes) {
+      const bool bit = (i >> 6) < ev.size() && ((ev[i >> 6] >> (i & 63)) & 1);
+      wrong += bit != (changed[i] != 0);
+    }
+    EXPECT(wrong == 0, "tree event: compute with call %d failing, then retried: %zu indices wrong", k, wrong);
+    const size_t bad = world_mismatches(tree, local, nodes);
+    EXPECT(bad == 0, "tree event: compute with call %d failing, then retried: %zu world matrices differ", k, bad);
+  }
+  EXPECT(threw > 5, "tree event: only %d injected failures reached compute", threw);
+  std::printf("  tree without relayout: %d of %d computes failed and recovered, events exact\n", threw, k);
+}
+
 struct Id : Payload {
   explicit Id(int v) : id(v) {}
   int id;
@@ -239,6 +291,7 @@ int main()
 {
   hook_counts_calls();
   tree_compute_failures();
+  tree_event_after_failures();
   manager_cull_failures();
   std::printf("fault injection: %d failures\n", failures);
   return failures ? 1 : 0;
