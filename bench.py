@@ -65,6 +65,8 @@ def parse():
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
     ap.add_argument("--serve-streams", type=int, default=8, help="streams of the serving measurement")
     ap.add_argument("--serve-depth", type=int, default=2, help="batches in flight per stream in the serving measurement")
+    ap.add_argument("--serve-direct", action="store_true",
+                    help="serving: the device reads pinned deltas directly instead of copying them")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="length of each serving-rate run")
     ap.add_argument("--tune", default="", help="batch config k,block,waves_per_cu,max_frusta")
     return ap.parse_args()
@@ -259,8 +261,9 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
     out_pool = PinnedArray((n_slots, slot_words), np.uint64)
     streams = getattr(args, "serve_streams", 8)
     depth = getattr(args, "serve_depth", 2)
+    direct = bool(getattr(args, "serve_direct", False))
     srv = Server(max_scenes=16, max_batch_requests=16, streams=streams, hbm_budget_bytes=32 << 30,
-                 batches_per_stream=depth)
+                 batches_per_stream=depth, direct_delta_reads=direct)
     ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scs]
     base_mats = [sc.matrices.reshape(-1, 16).copy() for sc in scs]
     # warm-up (untimed): requests through every output buffer, each carrying a
@@ -411,6 +414,7 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
         p.free()
     return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check, pinned_copy=pcie,
                 server={"streams": streams, "batches_per_stream": depth, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
+                        "deltas": "read directly from pinned memory" if direct else "copied (DMA)",
                         "scene_bytes": st["scene_bytes"], "peak_inflight_bytes": st["peak_inflight_bytes"]})
 
 

@@ -52,7 +52,7 @@ extern "C" {
 #define DPC_ERR_STATE 6
 #define DPC_ERR_INTERNAL 7 /* a launch schedule failed its own consistency check (a library bug) */
 
-#define DPC_ABI_VERSION 1
+#define DPC_ABI_VERSION 2
 
 /* ---- runtime plumbing (no reference counterpart: the reference's CPU path
  *      needs none, its GL sibling uses dp/gl buffers, opengl/src/GroupImpl.cpp:68-183) */
@@ -305,6 +305,9 @@ typedef struct dpc_server_config {
   uint32_t objects_per_lane;    /* kernel shape: 1 or 2 */
   uint32_t waves_per_cu;        /* kernel shape: persistent waves per CU */
   uint32_t batches_per_stream;  /* batches in flight per stream (0: 2, at most 4) */
+  uint32_t direct_delta_reads;  /* 1: a delta whose two arrays each lie in one pinned host
+                                   allocation is read by the device directly, with no
+                                   copy; 0: every delta is copied to the device first */
   uint64_t hbm_budget_bytes;    /* resident scenes + in-flight requests */
 } dpc_server_config;
 
@@ -348,10 +351,11 @@ int dpc_server_remove_scene(dpc_server* server, uint32_t scene_id);
  * indices) and delta_mats (n_delta x 16) are read while the request is being
  * launched; masks_out (at least n_frusta x ceil(n_objects / 64) words,
  * frustum-major, the dp::util::BitArray layout) receives the result.  All four
- * are borrowed until dpc_server_wait returns for the request.  Delta arrays that
- * each lie within one pinned host allocation (dpc_host_alloc) are read by the
- * device directly, with no copy; others are copied first.  A masks_out in pinned
- * memory receives a DMA transfer. */
+ * are borrowed until dpc_server_wait returns for the request.  With
+ * direct_delta_reads, delta arrays that each lie within one pinned host
+ * allocation (dpc_host_alloc) are read by the device directly, with no copy;
+ * others are copied first (a DMA transfer from pinned memory).  A masks_out in
+ * pinned memory receives a DMA transfer. */
 int dpc_server_submit(dpc_server* server, uint32_t scene_id, const float* vps, uint32_t n_frusta,
                       const uint32_t* delta_idx, const float* delta_mats, uint32_t n_delta, uint64_t* masks_out,
                       uint64_t mask_capacity_words, uint64_t* request_id);

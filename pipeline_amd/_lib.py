@@ -43,7 +43,7 @@ class CullPair(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("max_scenes", C.c_uint32), ("max_batch_requests", C.c_uint32), ("streams", C.c_uint32),
                 ("objects_per_lane", C.c_uint32), ("waves_per_cu", C.c_uint32), ("batches_per_stream", C.c_uint32),
-                ("hbm_budget_bytes", C.c_uint64)]
+                ("direct_delta_reads", C.c_uint32), ("hbm_budget_bytes", C.c_uint64)]
 
 
 class RequestInfo(C.Structure):

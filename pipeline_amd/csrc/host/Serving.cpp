@@ -14,6 +14,11 @@
 // scene of the batch against its request's frusta in one launch
 // (dpc::launch_serve), then copies each request's masks to its caller's
 // buffer.  Batches on different streams overlap on the device.
+//
+// The HIP calls of a batch are made by its stream's issuer thread, one per
+// stream, in the order the scheduler formed the stream's batches: a call that
+// blocks (the runtime's copy path can, for milliseconds) holds up that stream
+// only, never the scheduler or the other streams.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -88,11 +93,26 @@ struct StreamSlot {
   char* d_arena = nullptr;
   char* h_arena = nullptr;  // pinned, kHostArena bytes
   uint64_t cap = 0;
-  bool busy = false;
+  bool busy = false;     // formed and not yet completed
+  bool issued = false;   // its issuer has recorded the completion event
   std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<Scene> snapshot;  // the batch's scenes at forming (arrays stable while requests are in flight)
   double t_launch = 0;
   uint64_t bytes = 0;
   uint64_t tests = 0;
+};
+
+}  // namespace
+
+struct dpc_server;
+
+namespace {
+
+// The thread that makes the HIP calls of one stream's batches, in order.
+struct Issuer {
+  std::thread thread;
+  std::condition_variable cv;
+  std::deque<size_t> queue;  // slots to issue
 };
 
 }  // namespace
@@ -115,6 +135,8 @@ struct dpc_server {
   uint64_t arena_total = 0;
   bool stop = false;
   std::thread worker;
+  std::vector<std::unique_ptr<Issuer>> issuers;  // one per stream
+  bool issuers_stop = false;
   dpc_server_stats stats{};
   int device = 0;
 
@@ -123,7 +145,9 @@ struct dpc_server {
     return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
   }
   void run();
-  bool launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lock);
+  void issue_loop(int stream_index);
+  bool form_batch(size_t k);
+  void issue_batch(StreamSlot& slot);
   void complete(StreamSlot& slot);
 };
 
@@ -158,10 +182,11 @@ int hip_fail(hipError_t e, const char* what)
 
 }  // namespace
 
-// Forms and launches one batch on `slot`.  Called with the lock held; the HIP
-// calls run with it released (only the scheduler thread touches slots).
-bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lock)
+// Forms a batch on slot k and hands it to the slot's issuer.  Called by the
+// scheduler with the lock held.
+bool dpc_server::form_batch(size_t k)
 {
+  StreamSlot& slot = slots[k];
   std::vector<std::shared_ptr<Request>> pick;
   std::vector<bool> scene_taken(scenes.size(), false);
   uint64_t bytes = 0;
@@ -183,6 +208,7 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
   if (pick.empty()) return false;
   const uint32_t batch = ++batch_seq;
   const double t_admit = now();
+  slot.snapshot.clear();
   for (auto& r : pick) {
     scenes[r->scene].inflight++;
     scenes[r->scene].stream = slot.stream_index;
@@ -190,12 +216,25 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
     r->t_admit = t_admit;
     r->batch = batch;
     r->batch_requests = static_cast<uint32_t>(pick.size());
+    slot.snapshot.push_back(scenes[r->scene]);
   }
-  std::vector<Scene> snapshot;  // scene arrays are stable while requests are in flight
-  snapshot.reserve(pick.size());
-  for (auto& r : pick) snapshot.push_back(scenes[r->scene]);
-  lock.unlock();
+  slot.reqs = std::move(pick);
+  slot.busy = true;
+  slot.issued = false;
+  slot.t_launch = t_admit;
+  Issuer& is = *issuers[slot.stream_index];
+  is.queue.push_back(k);
+  is.cv.notify_one();
+  return true;
+}
+
+// Makes the HIP calls of the batch on `slot` (its issuer thread, without the
+// lock: the slot's batch belongs to the issuer until it is marked issued).
+void dpc_server::issue_batch(StreamSlot& slot)
+{
   dpcull::TraceRange trace("serve batch issue");
+  const std::vector<std::shared_ptr<Request>>& pick = slot.reqs;
+  const std::vector<Scene>& snapshot = slot.snapshot;
   const double t_issue0 = now();
 
   // Arena layout: the batch descriptors (entries, pairs, frusta; staged in the
@@ -301,25 +340,39 @@ bool dpc_server::launch_batch(StreamSlot& slot, std::unique_lock<std::mutex>& lo
     e = hipEventRecord(slot.done, s);
     if (e != hipSuccess) rc = hip_fail(e, "completion event");
   }
+  if (rc != DPC_OK) (void)hipStreamSynchronize(s);  // nothing may still read the arena when the batch ends
   const double t_issue1 = now();
   t_part[5] = t_issue1;
-  lock.lock();
+  std::lock_guard<std::mutex> lock(mu);
   stats.issue_us += t_issue1 - t_issue0;
   stats.issue_max_us = std::max(stats.issue_max_us, t_issue1 - t_issue0);
   for (int q = 0; q < 5; ++q) stats.issue_part_max_us[q] = std::max(stats.issue_part_max_us[q], t_part[q + 1] - t_part[q]);
-  slot.reqs = pick;
-  slot.busy = true;
-  slot.t_launch = t_admit;
   slot.bytes = end;
   slot.tests = tests;
   stats.peak_inflight_bytes = std::max<uint64_t>(stats.peak_inflight_bytes, end);
   if (rc != DPC_OK) {
-    // nothing reads the arena; finish the batch as failed once the stream drains
-    (void)hipStreamSynchronize(s);
-    for (auto& r : pick) r->status = rc;
+    for (auto& r : slot.reqs) r->status = rc;
     complete(slot);
+  } else {
+    slot.issued = true;
   }
-  return true;
+  cv_work.notify_all();  // the scheduler polls issued batches
+}
+
+void dpc_server::issue_loop(int stream_index)
+{
+  (void)hipSetDevice(device);
+  Issuer& is = *issuers[stream_index];
+  std::unique_lock<std::mutex> lock(mu);
+  for (;;) {
+    is.cv.wait(lock, [&] { return issuers_stop || !is.queue.empty(); });
+    if (is.queue.empty()) break;  // (stopping, and nothing left to issue)
+    const size_t k = is.queue.front();
+    is.queue.pop_front();
+    lock.unlock();
+    issue_batch(slots[k]);
+    lock.lock();
+  }
 }
 
 // Called with the lock held once the slot's completion event has fired.
@@ -342,7 +395,9 @@ void dpc_server::complete(StreamSlot& slot)
   stats.tests += slot.tests;
   stats.busy_us += t - slot.t_launch;
   slot.reqs.clear();
+  slot.snapshot.clear();
   slot.busy = false;
+  slot.issued = false;
   cv_done.notify_all();
 }
 
@@ -354,7 +409,9 @@ void dpc_server::run()
     bool progressed = false;
     bool any_busy = false;
     for (auto& slot : slots) {
-      if (slot.busy) {
+      if (slot.busy && !slot.issued) {
+        any_busy = true;  // its issuer is making the calls
+      } else if (slot.busy) {
         const hipError_t q = hipEventQuery(slot.done);
         if (q == hipSuccess) {
           complete(slot);
@@ -386,7 +443,7 @@ void dpc_server::run()
             best = k;
         if (best == slots.size()) break;
         tried[best] = true;
-        if (launch_batch(slots[best], lock)) {
+        if (form_batch(best)) {
           load[slots[best].stream_index]++;
           progressed = true;
           any_busy = true;
@@ -414,6 +471,7 @@ int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
   *out = nullptr;
   if (cfg->max_scenes == 0 || cfg->max_scenes > kMaxScenes || cfg->max_batch_requests == 0 ||
       cfg->max_batch_requests > kMaxBatch || cfg->streams == 0 || cfg->streams > 8 || cfg->batches_per_stream > 4 ||
+      cfg->direct_delta_reads > 1 ||
       (cfg->objects_per_lane != 1 && cfg->objects_per_lane != 2) || cfg->hbm_budget_bytes < (64ull << 20))
     return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_create: bad configuration");
   auto srv = std::make_unique<dpc_server>();
@@ -449,6 +507,11 @@ int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
     return rc;
   }
   dpc_server* s = srv.release();
+  for (uint32_t k = 0; k < cfg->streams; ++k) s->issuers.push_back(std::make_unique<Issuer>());
+  for (uint32_t k = 0; k < cfg->streams; ++k) {
+    const int index = static_cast<int>(k);
+    s->issuers[k]->thread = std::thread([s, index] { s->issue_loop(index); });
+  }
   s->worker = std::thread([s] { s->run(); });
   *out = s;
   return DPC_OK;
@@ -464,6 +527,15 @@ int dpc_server_destroy(dpc_server* s)
     }
     s->cv_work.notify_all();
     s->worker.join();
+  }
+  // the scheduler has ended with every batch completed: the issuers have nothing left
+  {
+    std::lock_guard<std::mutex> g(s->mu);
+    s->issuers_stop = true;
+  }
+  for (auto& is : s->issuers) {
+    is->cv.notify_all();
+    if (is->thread.joinable()) is->thread.join();
   }
   for (auto& st : s->streams)
     if (st) (void)hipStreamSynchronize(st);
@@ -589,7 +661,7 @@ int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32
   r->dmats = dmats;
   r->n_delta = n_delta;
   r->masks_out = masks_out;
-  if (n_delta && n_delta <= dpc::kMaxHostScatter) {
+  if (s->cfg.direct_delta_reads && n_delta && n_delta <= dpc::kMaxHostScatter) {
     const void* di = pinned_range(didx, 4ull * n_delta);
     const void* dm = di ? pinned_range(dmats, 64ull * n_delta) : nullptr;
     if (di && dm) {

@@ -67,10 +67,10 @@ class Server:
 
     def __init__(self, max_scenes: int = 16, max_batch_requests: int = 32, streams: int = 4,
                  objects_per_lane: int = 1, waves_per_cu: int = 0, hbm_budget_bytes: int = 16 << 30,
-                 batches_per_stream: int = 2):
+                 batches_per_stream: int = 2, direct_delta_reads: bool = False):
         self.lib = _lib.load()
         cfg = _lib.ServerConfig(max_scenes, max_batch_requests, streams, objects_per_lane, waves_per_cu,
-                                batches_per_stream, hbm_budget_bytes)
+                                batches_per_stream, int(direct_delta_reads), hbm_budget_bytes)
         self.handle = C.c_void_p()
         check(self.lib.dpc_server_create(C.byref(cfg), C.byref(self.handle)))
         self.scene_objects: dict[int, int] = {}

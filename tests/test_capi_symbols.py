@@ -29,7 +29,7 @@ def test_binding_matches_header():
 
 
 def test_abi_version_without_gpu(lib):
-    assert lib.dpc_abi_version() == 1
+    assert lib.dpc_abi_version() == 2
 
 
 def test_argument_errors_are_reported_before_any_device_call(lib):
