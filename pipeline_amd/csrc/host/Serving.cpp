@@ -458,7 +458,10 @@ void dpc_server::run()
       std::this_thread::yield();
       lock.lock();
     } else {
-      cv_work.wait_for(lock, std::chrono::microseconds(200));
+      // (a deadline on the system clock: libstdc++ waits for it with
+      // pthread_cond_timedwait, which ThreadSanitizer follows; a clock step
+      // only moves this re-check, since new work notifies the scheduler)
+      cv_work.wait_until(lock, std::chrono::system_clock::now() + std::chrono::microseconds(200));
     }
   }
 }

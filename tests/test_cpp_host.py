@@ -80,3 +80,30 @@ def test_host_mirrors_recover_from_failing_hip_calls():
     matrices vs the oracle; changed lists and visibility vs the CPU manager)."""
     out = run("fault_test")
     assert "fault injection: 0 failures" in out
+
+
+@pytest.mark.gpu
+def test_serve_concurrent_callers():
+    """The serving loop with three submitter threads (each owning some scenes,
+    waiting at once or later) and a statistics poller beside the scheduler and
+    the per-stream issuers: every request succeeds and its masks equal those of
+    the same requests run one at a time, for copied and directly read deltas."""
+    out = run("serve_stress")
+    assert "serve stress: 0 failures" in out
+
+
+def test_serving_threads_under_thread_sanitizer():
+    """The serving loop's host code (host/Serving.cpp: scheduler, per-stream
+    issuers, callers' submit/wait, a statistics poller) built with
+    ThreadSanitizer against a host-only runtime (tests/cpp/host_runtime: no
+    device, asynchronous calls complete at once, events a little later) and run
+    with the concurrent callers of serve_stress.cpp, on the CPU: every request
+    succeeds and ThreadSanitizer reports no data race."""
+    import os
+    subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp"), "tsan"], check=True, timeout=600)
+    env = dict(os.environ, TSAN_OPTIONS="halt_on_error=1:exitcode=66")
+    res = subprocess.run([str(BIN / "tsan_serve_host")], capture_output=True, text=True, timeout=600,
+                         cwd=str(ROOT), env=env)
+    assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-6000:]
+    assert "serve stress: 0 failures" in res.stdout
+    assert "ThreadSanitizer" not in res.stderr, res.stderr[-6000:]
