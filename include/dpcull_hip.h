@@ -354,8 +354,10 @@ int dpc_server_remove_scene(dpc_server* server, uint32_t scene_id);
  * are borrowed until dpc_server_wait returns for the request.  With
  * direct_delta_reads, delta arrays that each lie within one pinned host
  * allocation (dpc_host_alloc) are read by the device directly, with no copy;
- * others are copied first (a DMA transfer from pinned memory).  A masks_out in
- * pinned memory receives a DMA transfer. */
+ * others are copied first (a DMA transfer from pinned memory).  A masks_out
+ * whose mask_capacity_words lie within one pinned host allocation is written by
+ * the device directly (one copy-out launch per batch, no DMA transfer); any
+ * other masks_out receives a copy. */
 int dpc_server_submit(dpc_server* server, uint32_t scene_id, const float* vps, uint32_t n_frusta,
                       const uint32_t* delta_idx, const float* delta_mats, uint32_t n_delta, uint64_t* masks_out,
                       uint64_t mask_capacity_words, uint64_t* request_id);

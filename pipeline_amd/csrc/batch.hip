@@ -551,6 +551,30 @@ int dpc_cull_batch(const dpc_batch_plan* plan, const float* d_vps, uint64_t* d_m
 namespace dpc {
 // Launches one serving batch (see ServeEntry).  d_entries/d_pairs/d_vps/d_masks
 // live on the device; total_cost and n_entries come from the host copy.
+// Copy-out of a serving batch's masks: blockIdx.y = the request, threads take
+// consecutive 64-bit words (a wave writes 512 contiguous bytes per store).
+// The destinations are pinned host memory, which device stores reach directly.
+__global__ __launch_bounds__(256) void copy_out_kernel(const CopyOut* __restrict__ copies, const char* __restrict__ arena)
+{
+  const CopyOut c = copies[blockIdx.y];
+  const auto* src = reinterpret_cast<const unsigned long long*>(arena + c.src_offset);
+  for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; w < c.words;
+       w += static_cast<uint64_t>(gridDim.x) * 256)
+    c.dst[w] = src[w];
+}
+
+int launch_copy_out(const void* d_copies, uint32_t n, const char* d_arena, uint64_t max_words, void* stream_handle)
+{
+  if (n == 0 || max_words == 0) return DPC_OK;
+  if (n > 65535) return set_error(DPC_ERR_INVALID_ARGUMENT, "launch_copy_out: more than 65535 ranges");
+  // enough blocks per range for its words, at most 64 (then each thread loops)
+  const uint64_t per = (max_words + 255) / 256;
+  const unsigned bx = static_cast<unsigned>(per < 64 ? per : 64);
+  hipLaunchKernelGGL(copy_out_kernel, dim3(bx, n), dim3(256), 0, as_stream(stream_handle),
+                     static_cast<const CopyOut*>(d_copies), d_arena);
+  return check_launch("copy_out_kernel");
+}
+
 int launch_serve(const void* d_entries, uint32_t n_entries, uint64_t total_cost, const void* d_pairs,
                  const void* d_scenes, const float* d_vps, uint64_t* d_masks, uint32_t objects_per_lane,
                  uint32_t waves_per_cu, void* stream_handle)

@@ -42,4 +42,20 @@ constexpr uint32_t kMaxHostScatterLd = (1u << 26) - 1;
 int launch_scatter_host(float* d_planes, size_t ld, const uint32_t* idx, const float* src, size_t count,
                         void* stream);
 
+// One request's masks to its caller's buffer in pinned host memory: `words`
+// 64-bit words from byte offset src_offset of the batch arena to dst (the
+// buffer's device-visible address).
+struct CopyOut {
+  uint64_t src_offset;
+  uint64_t words;
+  uint64_t* dst;
+  uint64_t pad;
+};
+static_assert(sizeof(CopyOut) == 32, "CopyOut layout");
+
+// Copies n CopyOut ranges (d_copies: device memory) out of the arena in one
+// launch; the device writes the callers' pinned buffers itself, in 512-byte
+// wave-wide stores, instead of a DMA transfer per request.
+int launch_copy_out(const void* d_copies, uint32_t n, const char* d_arena, uint64_t max_words, void* stream);
+
 }  // namespace dpc

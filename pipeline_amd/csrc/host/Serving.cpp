@@ -46,8 +46,8 @@ namespace {
 constexpr uint32_t kMaxScenes = 64;
 constexpr uint32_t kMaxBatch = 64;
 constexpr uint32_t kMaxFrusta = 64;
-// the pinned staging area of one batch's descriptors (entries, pairs, frusta)
-constexpr uint64_t kHostArena = 4096 + kMaxBatch * 32 + kMaxBatch * kMaxFrusta * (16 + 64) + 3 * 256;
+// the pinned staging area of one batch's descriptors (entries, pairs, frusta, copy-outs)
+constexpr uint64_t kHostArena = 4096 + kMaxBatch * 32 + kMaxBatch * kMaxFrusta * (16 + 64) + kMaxBatch * 32 + 4 * 256;
 
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
@@ -77,6 +77,7 @@ struct Request {
   const uint32_t* didx_dev = nullptr;
   const float* dmats_dev = nullptr;
   uint64_t* masks_out = nullptr;
+  uint64_t* masks_dev = nullptr;  // masks_out lies in pinned host memory: the device writes it, at this address
   uint64_t words = 0;       // mask words per frustum
   uint64_t dev_bytes = 0;   // arena bytes this request needs
   double t_submit = 0, t_admit = 0, t_done = 0;
@@ -252,6 +253,8 @@ void dpc_server::issue_batch(StreamSlot& slot)
   off = align_up(off + sizeof(dpc::KPair) * n_pairs, 256);
   const uint64_t vps_off = off;
   off = align_up(off + 64ull * n_pairs, 256);
+  const uint64_t copies_off = off;
+  off = align_up(off + sizeof(dpc::CopyOut) * pick.size(), 256);
   const uint64_t desc_end = off;
   std::vector<uint64_t> idx_off(pick.size()), mat_off(pick.size());
   for (size_t k = 0; k < pick.size(); ++k) {
@@ -266,6 +269,9 @@ void dpc_server::issue_batch(StreamSlot& slot)
   auto* entries = reinterpret_cast<dpc::ServeEntry*>(slot.h_arena + entries_off);
   auto* pairs = reinterpret_cast<dpc::KPair*>(slot.h_arena + pairs_off);
   auto* vps = reinterpret_cast<float*>(slot.h_arena + vps_off);
+  auto* copies = reinterpret_cast<dpc::CopyOut*>(slot.h_arena + copies_off);
+  uint32_t n_copies = 0;
+  uint64_t copy_words = 0;
   std::vector<uint64_t> mask_off(pick.size());  // byte offset of each request's masks in the device arena
   uint64_t cost = 0, mask_word = 0, tests = 0;
   uint32_t p = 0;
@@ -289,6 +295,10 @@ void dpc_server::issue_batch(StreamSlot& slot)
       std::memcpy(vps + 16ull * p, r.vps + 16ull * f, 64);
     }
     tests += sc.n * r.n_frusta;
+    if (r.masks_dev) {
+      copies[n_copies++] = dpc::CopyOut{mask_off[k], r.words * r.n_frusta, r.masks_dev, 0};
+      copy_words = std::max<uint64_t>(copy_words, r.words * r.n_frusta);
+    }
   }
   const uint64_t end = masks_off + 8 * mask_word;
   int rc = DPC_OK;
@@ -330,9 +340,12 @@ void dpc_server::issue_batch(StreamSlot& slot)
                            reinterpret_cast<uint64_t*>(slot.d_arena + masks_off), K, cfg.waves_per_cu, s);
   }
   t_part[4] = now();
-  // each request's masks go straight to the caller's buffer
+  // each request's masks go straight to the caller's buffer: written by the
+  // device in one launch for every buffer in pinned memory, copied otherwise
+  if (rc == DPC_OK) rc = dpc::launch_copy_out(slot.d_arena + copies_off, n_copies, slot.d_arena, copy_words, s);
   for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
     const Request& r = *pick[k];
+    if (r.masks_dev) continue;
     e = hipMemcpyAsync(r.masks_out, slot.d_arena + mask_off[k], 8 * r.words * r.n_frusta, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) rc = hip_fail(e, "mask download");
   }
@@ -664,6 +677,8 @@ int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32
   r->dmats = dmats;
   r->n_delta = n_delta;
   r->masks_out = masks_out;
+  if (mask_capacity_words && mask_capacity_words <= (~0ull >> 4))
+    r->masks_dev = static_cast<uint64_t*>(const_cast<void*>(pinned_range(masks_out, 8 * mask_capacity_words)));
   if (s->cfg.direct_delta_reads && n_delta && n_delta <= dpc::kMaxHostScatter) {
     const void* di = pinned_range(didx, 4ull * n_delta);
     const void* dm = di ? pinned_range(dmats, 64ull * n_delta) : nullptr;
@@ -688,8 +703,10 @@ int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32
   if (mask_capacity_words < r->words * n_frusta)
     return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: the mask buffer is smaller than n_frusta x ceil(n/64) words");
   const uint64_t staged_delta = r->didx_dev ? 0 : n_delta;  // a pinned delta takes no arena space
+  // (the last term: a batch's alignment to 256 bytes of its four descriptor
+  // regions and of this request's two delta regions)
   r->dev_bytes = align_up(4ull * staged_delta, 256) + align_up(64ull * staged_delta, 256) + sizeof(dpc::ServeEntry) +
-                 (sizeof(dpc::KPair) + 64) * n_frusta + 8 * r->words * n_frusta + 1024;
+                 sizeof(dpc::CopyOut) + (sizeof(dpc::KPair) + 64) * n_frusta + 8 * r->words * n_frusta + 6 * 256;
   if (r->dev_bytes > s->slots[0].cap)
     return fail(DPC_ERR_OVER_BUDGET, "dpc_server_submit: the request alone exceeds the in-flight budget");
   r->id = s->next_id++;

@@ -154,6 +154,7 @@ int launch_serve(const void*, uint32_t, uint64_t, const void*, const void*, cons
   return DPC_OK;
 }
 int launch_scatter_host(float*, size_t, const uint32_t*, const float*, size_t, void*) { return DPC_OK; }
+int launch_copy_out(const void*, uint32_t, const char*, uint64_t, void*) { return DPC_OK; }
 }  // namespace dpc
 
 extern "C" {

@@ -48,6 +48,12 @@ struct Req {
   uint32_t n_delta = 0;
   bool pinned = false;
   std::vector<uint64_t> masks;
+  uint64_t* pinned_masks = nullptr;  // every other request: its masks go to pinned memory (written by the device)
+  uint64_t* out() { return pinned_masks ? pinned_masks : masks.data(); }
+  void collect()  // the masks as the server delivered them
+  {
+    if (pinned_masks) std::memcpy(masks.data(), pinned_masks, masks.size() * 8);
+  }
 };
 
 static void identity_into(float* m)
@@ -121,6 +127,14 @@ static std::vector<Req> make_requests(std::mt19937& rng, const std::vector<Scene
       }
     }
     r.masks.assign(r.n_frusta * ((sc.n + 63) / 64), 0);
+    if (k % 2 == 0) {
+      void* pm = nullptr;
+      if (dpc_host_alloc(&pm, r.masks.size() * 8) != DPC_OK) {
+        std::printf("FAIL: dpc_host_alloc: %s\n", dpc_last_error());
+        std::exit(1);
+      }
+      r.pinned_masks = static_cast<uint64_t*>(pm);
+    }
   }
   return out;
 }
@@ -128,6 +142,7 @@ static std::vector<Req> make_requests(std::mt19937& rng, const std::vector<Scene
 static void free_requests(std::vector<Req>& reqs)
 {
   for (auto& r : reqs) {
+    if (r.pinned_masks) dpc_host_free(r.pinned_masks);
     if (!r.n_delta) continue;
     if (r.pinned) {
       dpc_host_free(r.idx);
@@ -184,12 +199,13 @@ static void run(uint32_t direct)
     for (size_t k = 0; k < reqs.size(); ++k) {
       Req& r = reqs[k];
       uint64_t id = 0;
-      int rc = dpc_server_submit(s, ids[r.scene], r.vps.data(), r.n_frusta, r.idx, r.mats, r.n_delta,
-                                 r.masks.data(), r.masks.size(), &id);
+      int rc = dpc_server_submit(s, ids[r.scene], r.vps.data(), r.n_frusta, r.idx, r.mats, r.n_delta, r.out(),
+                                 r.masks.size(), &id);
       EXPECT(rc == DPC_OK, "submit: %s", dpc_last_error());
       dpc_request_info info{};
       rc = dpc_server_wait(s, id, -1.0, &info);
       EXPECT(rc == DPC_OK && info.status == DPC_OK, "wait: %d", rc);
+      r.collect();
       want[k] = r.masks;
     }
     dpc_server_destroy(s);
@@ -199,7 +215,10 @@ static void run(uint32_t direct)
   // scene's requests keep their order
   dpc_server* s = make_server(4, direct);
   const auto ids = add_scenes(s, scenes);
-  for (auto& r : reqs) std::fill(r.masks.begin(), r.masks.end(), 0);
+  for (auto& r : reqs) {
+    std::fill(r.masks.begin(), r.masks.end(), 0);
+    if (r.pinned_masks) std::memset(r.pinned_masks, 0, r.masks.size() * 8);
+  }
   std::atomic<bool> done{false};
   std::thread poller([&] {
     dpc_server_stats st{};
@@ -211,26 +230,28 @@ static void run(uint32_t direct)
   std::vector<std::thread> submitters;
   for (int t = 0; t < 3; ++t) {
     submitters.emplace_back([&, t] {
-      std::vector<uint64_t> later;
+      std::vector<std::pair<uint64_t, size_t>> later;
       for (size_t k = 0; k < reqs.size(); ++k) {
         Req& r = reqs[k];
         if (r.scene % 3 != t) continue;
         uint64_t id = 0;
         const int rc = dpc_server_submit(s, ids[r.scene], r.vps.data(), r.n_frusta, r.idx, r.mats, r.n_delta,
-                                         r.masks.data(), r.masks.size(), &id);
+                                         r.out(), r.masks.size(), &id);
         EXPECT(rc == DPC_OK, "submit: %s", dpc_last_error());
-        if (k % 2) {
-          later.push_back(id);
+        if (k % 3 == 1) {
+          later.emplace_back(id, k);
         } else {
           dpc_request_info info{};
           const int w = dpc_server_wait(s, id, -1.0, &info);
           EXPECT(w == DPC_OK && info.status == DPC_OK, "wait: %d", w);
+          r.collect();
         }
       }
-      for (uint64_t id : later) {
+      for (auto [id, k] : later) {
         dpc_request_info info{};
         const int w = dpc_server_wait(s, id, -1.0, &info);
         EXPECT(w == DPC_OK && info.status == DPC_OK, "wait later: %d", w);
+        reqs[k].collect();
       }
     });
   }

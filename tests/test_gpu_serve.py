@@ -35,18 +35,30 @@ class Replay:
         return np.concatenate([self.orc.cull_obbs(obbs, vp) for vp in req.vps])
 
 
-def _run_mix(scenes, reqs, streams=3, max_batch=3, direct_delta_reads=False):
-    out = []
-    with Server(max_scenes=16, max_batch_requests=max_batch, streams=streams, hbm_budget_bytes=4 << 30,
-                direct_delta_reads=direct_delta_reads) as srv:
-        ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scenes]
-        pending = []
-        for r in reqs:
-            m = np.zeros(srv.mask_words(ids[r.scene], r.vps.shape[0]), np.uint64)
-            pending.append((srv.submit(ids[r.scene], r.vps, m, r.delta_idx, r.delta_mats), m))
-        for rid, m in pending:
-            out.append((srv.wait(rid, 60_000.0), m))
-        stats = srv.stats()
+def _run_mix(scenes, reqs, streams=3, max_batch=3, direct_delta_reads=False, pinned_masks=False):
+    """Every request's (info, masks); with pinned_masks, every other request's
+    masks go to a buffer in pinned memory, which the device writes itself."""
+    out, held = [], []
+    try:
+        with Server(max_scenes=16, max_batch_requests=max_batch, streams=streams, hbm_budget_bytes=4 << 30,
+                    direct_delta_reads=direct_delta_reads) as srv:
+            ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scenes]
+            pending = []
+            for i, r in enumerate(reqs):
+                words = srv.mask_words(ids[r.scene], r.vps.shape[0])
+                if pinned_masks and i % 2 == 0:
+                    pa = PinnedArray((words,), np.uint64)
+                    held.append(pa)
+                    m = pa.array
+                else:
+                    m = np.zeros(words, np.uint64)
+                pending.append((srv.submit(ids[r.scene], r.vps, m, r.delta_idx, r.delta_mats), m))
+            for rid, m in pending:
+                out.append((srv.wait(rid, 60_000.0), np.array(m, copy=True)))
+            stats = srv.stats()
+    finally:
+        for pa in held:
+            pa.free()
     return out, stats
 
 
@@ -105,7 +117,8 @@ def test_serve_pinned_deltas_read_directly(orc, direct_reads):
     """With direct_delta_reads, deltas whose two arrays lie in pinned host
     memory are read by the device directly (no copy); mixed in one batch with
     copied deltas, every request's masks still equal the oracle's replay.
-    Without it, the same pinned deltas are copied."""
+    Without it, the same pinned deltas are copied.  Every other request's masks
+    go to pinned memory, which the device writes itself (the others are copied)."""
     scenes = S.serving_scenes(seed=11, n_scenes=6, size_range=(3_000, 120_000))
     reqs = S.serving_requests(scenes, seed=12, n_requests=48, rate_per_s=0.0)
     pinned, mixed, direct = [], [], 0
@@ -124,7 +137,8 @@ def test_serve_pinned_deltas_read_directly(orc, direct_reads):
                 mixed.append(S.ServingRequest(r.scene, r.vps, pi.array, r.delta_mats, r.arrival_us))
         else:
             mixed.append(r)
-    mix, stats = _run_mix(scenes, mixed, direct_delta_reads=direct_reads)
+    # one stream: the requests queue behind its two arenas and meet in batches
+    mix, stats = _run_mix(scenes, mixed, streams=1, direct_delta_reads=direct_reads, pinned_masks=True)
     assert all(info.status == 0 for info, _ in mix)
     assert direct > 0 and stats["delta_direct"] == (direct if direct_reads else 0)
     assert max(info.batch_requests for info, _ in mix) > 1
@@ -325,8 +339,16 @@ def test_serve_sixteen_resident_scenes_poisson(orc, direct_reads):
                         timeline.append(S.ServingRequest(r.scene, r.vps, pi[o1:o1 + k], pm[o2:o2 + k], r.arrival_us))
                     else:
                         timeline.append(r)
-                results, masks = _submit_on_schedule(
-                    srv, ids, timeline, lambda i, r: np.zeros(srv.mask_words(ids[r.scene], r.vps.shape[0]), np.uint64))
+                def pick_masks(i, r):
+                    words = srv.mask_words(ids[r.scene], r.vps.shape[0])
+                    if i % 2:  # pinned: the device writes it
+                        pa = PinnedArray((words,), np.uint64)
+                        pinned.append(pa)
+                        return pa.array
+                    return np.zeros(words, np.uint64)
+
+                results, masks = _submit_on_schedule(srv, ids, timeline, pick_masks)
+                masks = [np.array(m, copy=True) for m in masks]  # (the pinned buffers are freed below)
                 # the replay below reads the original (pageable) views of the same rows:
                 # the pinned copies are freed before it runs
                 rate_runs.append((rate, reqs, results, masks))
