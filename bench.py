@@ -166,7 +166,28 @@ def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
     b = 16 * words + 4 * flip.size
     out["result_diff"] = dict(bits=m, changed=count, changed_expected=int(flip.size), us=k_ms * 1e3, bytes=b,
                               gbs=b / (k_ms * 1e-3) / 1e9)
-    for d in (d_old, d_idx, d_cnt, ws):
+    ws.free()
+    # the tree objects' cull alone, and the cull with the result diff against the
+    # same old mask fused into it (what Manager::cull runs per frame)
+    d_mask = dev.DeviceArray(words, np.uint64)
+    ws = dev.workspace(lib.dpc_cull_diff_workspace(m))
+    vp = dev.vp_arg(ts.vp)
+
+    def issue_cull():
+        _lib.check(lib.dpc_cull(objs.box_a.ptr, objs.box_b.ptr, objs.xform_addr, m, d_world.ptr, n_slots, vp,
+                                d_mask.ptr, stream.handle))
+
+    def issue_cull_diff():
+        _lib.check(lib.dpc_cull_diff(objs.box_a.ptr, objs.box_b.ptr, objs.xform_addr, m, d_world.ptr, n_slots, vp,
+                                     d_mask.ptr, d_old.ptr, d_idx.ptr, d_cnt.ptr, 0, ws.ptr, stream.handle))
+
+    _, c_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_cull)
+    _, cd_ms, _ = run_timed(dev, lib, stream, 50, 5, issue_cull_diff)
+    count_fused = int(d_cnt.download(stream.handle)[0])
+    same = bool(np.array_equal(d_mask.download(stream.handle), new_h))
+    out["cull_with_result_diff"] = dict(objects=m, cull_us=c_ms * 1e3, fused_us=cd_ms * 1e3,
+                                        diff_us=(cd_ms - c_ms) * 1e3, changed=count_fused, mask_equal=same)
+    for d in (d_old, d_idx, d_cnt, ws, d_mask):
         d.free()
     # inverse transpose of every world matrix (dp::math::invertTranspose)
     d_inv = dev.DeviceArray(4 * n_slots * 4, np.float32)
@@ -202,6 +223,10 @@ def bench_next_rows(dev, lib, stream, ts, objs, d_local, d_world, tplan, tmask):
     out["dirty_propagation_all"] = dict(nodes=n_slots - 1, us=k_ms * 1e3)
     d_dl.free()
     d_dw.free()
+    cd = out["cull_with_result_diff"]
+    log("next rows: cull {:.1f} us, cull with the result diff fused {:.1f} us (+{:.1f} us, {} changed, mask "
+        "{})".format(cd["cull_us"], cd["fused_us"], cd["diff_us"], cd["changed"],
+                     "equal" if cd["mask_equal"] else "DIFFERS"))
     log("next rows: bbox {:.1f} us ({:.0f} GB/s), result diff {:.1f} us ({} of {} changed), inverse transpose "
         "{:.1f} us ({:.0f} GB/s), propagation with 1 % dirty {:.1f} us ({} world matrices changed)".format(
             out["bbox"]["us"], out["bbox"]["gbs"], out["result_diff"]["us"], out["result_diff"]["changed"],

@@ -276,6 +276,20 @@ int dpc_result_diff(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint3
 int dpc_result_diff_noalloc(uint64_t* d_old, const uint64_t* d_new, size_t n_bits, uint32_t* d_changed_idx,
                             uint32_t* d_count, int update_old, void* stream);
 
+/* dpc_cull followed by dpc_result_diff of its mask against d_old: the tail of
+ * cpu::ManagerImpl::cull, updateChanged(visible) (dp/culling/cpu/src/ManagerImpl.cpp:517,
+ * dp/culling/src/ResultBitSet.cpp:61-108).  d_mask receives the visibility bits
+ * as from dpc_cull; d_changed_idx, *d_count and (with update_old) d_old are
+ * then as from dpc_result_diff(d_old, d_mask, n_objects, ...).  The cull also
+ * counts each word's changed bits, so up to 2^22 objects this takes two
+ * launches instead of three (above that, the result diff's own launches
+ * follow the cull).  d_old and d_mask must not overlap; d_workspace needs
+ * dpc_cull_diff_workspace(n_objects) bytes, 16-byte aligned. */
+size_t dpc_cull_diff_workspace(size_t n_objects);
+int dpc_cull_diff(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+                  const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, uint64_t* d_old,
+                  uint32_t* d_changed_idx, uint32_t* d_count, int update_old, void* d_workspace, void* stream);
+
 /* ---- group bounding box (ManagerBitSet::calculateBoundingBox, dp/culling/src/ManagerBitSet.cpp:268-306)
  * World-space AABB of every object's eight transformed corners; d_out6 =
  * lower.xyz, upper.xyz as the reference's Box3f(min, max) returns them.

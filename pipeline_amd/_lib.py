@@ -124,6 +124,8 @@ _SIGNATURES = {
     "dpc_result_diff_workspace": (SZ, [SZ]),
     "dpc_result_diff": (I, [P, P, SZ, P, P, I, P, P]),
     "dpc_result_diff_noalloc": (I, [P, P, SZ, P, P, I, P]),
+    "dpc_cull_diff_workspace": (SZ, [SZ]),
+    "dpc_cull_diff": (I, [P, P, P, SZ, P, SZ, FP, P, P, P, P, I, P, P]),
     "dpc_bbox_workspace": (SZ, [SZ]),
     "dpc_bbox": (I, [P, P, P, SZ, P, SZ, P, P, P]),
     "dpc_inverse_transpose": (I, [P, SZ, P, SZ, P, SZ, P, P]),

@@ -21,6 +21,19 @@ namespace {
 constexpr unsigned kCullThreads = 256;  // persistent streaming form; plain form with an index array
 constexpr unsigned kPlainThreads = 1024;  // plain form without an index array
 
+// With Diff (the cull of Manager::cull, which the result diff follows), the
+// wave that stores mask word w also stores the number of bits in which it
+// differs from old[w] (bits below n only) as count[w], one byte per word:
+// the write pass of the diff then needs no counting launch of its own.
+__device__ __forceinline__ uint8_t changed_in_word(unsigned long long old_bits, unsigned long long new_bits, size_t w,
+                                                   size_t n)
+{
+  unsigned long long x = old_bits ^ new_bits;
+  const size_t tail = n & 63;
+  if (tail && w == (n >> 6)) x &= (~0ull) >> (64 - tail);
+  return static_cast<uint8_t>(__popcll(x));
+}
+
 // Plain form: one object per lane, one 64-object tile per wave, a grid that
 // covers all tiles.  The object's four matrix rows come from four row planes,
 // so every load of the wave is one contiguous 1 KiB (rows, box_a) or 512 B
@@ -29,20 +42,27 @@ constexpr unsigned kPlainThreads = 1024;  // plain form without an index array
 // they are loaded as streaming data in 1024-thread blocks; with one (matrices
 // shared by several objects, e.g. the leaves of a transform tree), plain cached
 // loads in 256-thread blocks measured faster.
-template <bool Indexed>
+template <bool Indexed, bool Diff>
 __global__ __launch_bounds__(Indexed ? kCullThreads : kPlainThreads) void cull_kernel(const float4* __restrict__ box_a,
                                                             const float2* __restrict__ box_b,
                                                             const uint32_t* __restrict__ xform, size_t n,
                                                             const float4* __restrict__ planes, size_t ld, Mat16 vp,
-                                                            unsigned long long* __restrict__ mask)
+                                                            unsigned long long* __restrict__ mask,
+                                                            const unsigned long long* __restrict__ old,
+                                                            uint8_t* __restrict__ count)
 {
   const size_t i = static_cast<size_t>(blockIdx.x) * (Indexed ? kCullThreads : kPlainThreads) + threadIdx.x;
+  const bool writer = (threadIdx.x & (kWave - 1)) == 0 && i < n;  // lane 0 of a wave holding objects
+  const unsigned long long old_bits = Diff && writer ? old[i >> 6] : 0ull;  // loaded with the objects
   const Slot s = make_slot(i, n, Indexed ? xform : nullptr, ld);
   const RawObj r = Indexed ? load_raw(box_a, box_b, planes, ld, s.i, s.m)
                            : load_raw_stream<true>(box_a, box_b, planes, ld, s.i, s.m);
   const bool vis = slot_visible(s, obb_visible(obb_of(r), vp.m));
   const unsigned long long bits = __ballot(vis);
-  if ((threadIdx.x & (kWave - 1)) == 0 && i < n) mask[i >> 6] = bits;
+  if (writer) {
+    mask[i >> 6] = bits;
+    if (Diff) count[i >> 6] = changed_in_word(old_bits, bits, i >> 6, n);
+  }
 }
 
 // Streaming form: a persistent grid in which every wave walks tiles t, t+W,
@@ -50,33 +70,42 @@ __global__ __launch_bounds__(Indexed ? kCullThreads : kPlainThreads) void cull_k
 // next one are in flight.  Every wave runs the same number of steps (tiles past
 // the end load clamped indices and store nothing), so all loads are
 // unconditional and the wait before each compute covers only that tile's loads.
-template <bool Indexed>
+template <bool Indexed, bool Diff>
 __device__ __forceinline__ void stream_fetch(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
                                              const uint32_t* __restrict__ xform, size_t n,
                                              const float4* __restrict__ planes, size_t ld, size_t tile, unsigned lane,
-                                             Slot& s, RawObj& r)
+                                             const unsigned long long* __restrict__ old, Slot& s, RawObj& r,
+                                             unsigned long long& old_bits)
 {
   s = make_slot(tile * kWave + lane, n, Indexed ? xform : nullptr, ld);
   r = load_raw(box_a, box_b, planes, ld, s.i, s.m);
+  old_bits = Diff && lane == 0 && tile * kWave < n ? old[tile] : 0ull;
   // keep the loads issued here, ahead of the test of the previous tile (the
   // scheduler would otherwise sink them below it to shorten register lifetimes)
   __builtin_amdgcn_sched_barrier(0);
 }
 
-__device__ __forceinline__ void stream_test(const Slot& s, const RawObj& r, const Mat16& vp, size_t tile,
-                                            size_t n_tiles, unsigned lane, unsigned long long* __restrict__ mask)
+template <bool Diff>
+__device__ __forceinline__ void stream_test(const Slot& s, const RawObj& r, const Mat16& vp, size_t tile, size_t n,
+                                            size_t n_tiles, unsigned lane, unsigned long long* __restrict__ mask,
+                                            unsigned long long old_bits, uint8_t* __restrict__ count)
 {
   const bool vis = slot_visible(s, obb_visible(obb_of(r), vp.m));
   const unsigned long long bits = __ballot(vis);
-  if (lane == 0 && tile < n_tiles) mask[tile] = bits;
+  if (lane == 0 && tile < n_tiles) {
+    mask[tile] = bits;
+    if (Diff) count[tile] = changed_in_word(old_bits, bits, tile, n);
+  }
 }
 
-template <bool Indexed>
+template <bool Indexed, bool Diff>
 __global__ __launch_bounds__(kCullThreads) void cull_stream_kernel(const float4* __restrict__ box_a,
                                                                    const float2* __restrict__ box_b,
                                                                    const uint32_t* __restrict__ xform, size_t n,
                                                                    const float4* __restrict__ planes, size_t ld,
-                                                                   Mat16 vp, unsigned long long* __restrict__ mask)
+                                                                   Mat16 vp, unsigned long long* __restrict__ mask,
+                                                                   const unsigned long long* __restrict__ old,
+                                                                   uint8_t* __restrict__ count)
 {
   const unsigned lane = threadIdx.x & (kWave - 1);
   const size_t waves_per_block = blockDim.x / kWave;
@@ -86,13 +115,14 @@ __global__ __launch_bounds__(kCullThreads) void cull_stream_kernel(const float4*
   size_t t = static_cast<size_t>(blockIdx.x) * waves_per_block + threadIdx.x / kWave;
   Slot sa, sb;
   RawObj ra, rb;
-  stream_fetch<Indexed>(box_a, box_b, xform, n, planes, ld, t, lane, sa, ra);
+  unsigned long long oa, ob;
+  stream_fetch<Indexed, Diff>(box_a, box_b, xform, n, planes, ld, t, lane, old, sa, ra, oa);
   for (size_t k = 0; k < steps; k += 2) {
-    stream_fetch<Indexed>(box_a, box_b, xform, n, planes, ld, t + W, lane, sb, rb);
-    stream_test(sa, ra, vp, t, n_tiles, lane, mask);
+    stream_fetch<Indexed, Diff>(box_a, box_b, xform, n, planes, ld, t + W, lane, old, sb, rb, ob);
+    stream_test<Diff>(sa, ra, vp, t, n, n_tiles, lane, mask, oa, count);
     if (k + 1 >= steps) break;
-    stream_fetch<Indexed>(box_a, box_b, xform, n, planes, ld, t + 2 * W, lane, sa, ra);
-    stream_test(sb, rb, vp, t + W, n_tiles, lane, mask);
+    stream_fetch<Indexed, Diff>(box_a, box_b, xform, n, planes, ld, t + 2 * W, lane, old, sa, ra, oa);
+    stream_test<Diff>(sb, rb, vp, t + W, n, n_tiles, lane, mask, ob, count);
     t += 2 * W;
   }
 }
@@ -436,6 +466,24 @@ __device__ __forceinline__ unsigned long long diff_word(const unsigned long long
   return x;
 }
 
+// diff_word without a branch: both words are loaded (from a clamped index)
+// on every lane, so the loads of a kernel's later steps can be issued before
+// anything waits for these.
+__device__ __forceinline__ unsigned long long diff_word_clamped(const unsigned long long* old_w,
+                                                                const unsigned long long* new_w, size_t w,
+                                                                size_t n_words, size_t n_bits,
+                                                                unsigned long long& new_bits)
+{
+  const size_t c = w < n_words ? w : n_words - 1;  // n_words >= 1 wherever a diff kernel runs
+  new_bits = new_w[c];
+  unsigned long long x = old_w[c] ^ new_bits;
+  const size_t tail = n_bits & 63;
+  const unsigned long long keep = (tail && c == n_words - 1) ? (~0ull) >> (64 - tail) : ~0ull;
+  return w < n_words ? (x & keep) : 0ull;
+}
+
+// Exclusive scan over a block of Threads threads (lds: Threads / 64 + 1 words).
+template <unsigned Threads = kDiffThreads>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds, uint32_t* total)
 {
   // Wave-level inclusive scan, then scan of the wave totals.
@@ -449,16 +497,16 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t run = 0;
-    for (unsigned k = 0; k < blockDim.x / kWave; ++k) {
+    for (unsigned k = 0; k < Threads / kWave; ++k) {
       const uint32_t t = lds[k];
       lds[k] = run;
       run += t;
     }
-    lds[blockDim.x / kWave] = run;
+    lds[Threads / kWave] = run;
   }
   __syncthreads();
   const uint32_t r = lds[wave] + x - v;
-  *total = lds[blockDim.x / kWave];
+  *total = lds[Threads / kWave];
   __syncthreads();
   return r;
 }
@@ -525,12 +573,23 @@ __global__ __launch_bounds__(kDiffThreads) void diff_write_direct_kernel(unsigne
                                                                          uint32_t* __restrict__ d_count)
 {
   __shared__ uint32_t lds[kDiffThreads / kWave + 1];
-  uint32_t before = 0;  // this thread's share of the counts of blocks [0, blockIdx.x)
-  for (unsigned k = threadIdx.x; k < blockIdx.x; k += kDiffThreads) before += block_count[k];
+  // this thread's words first: their loads are then in flight with those of the counts
+  const size_t w = static_cast<size_t>(blockIdx.x) * kDiffThreads + threadIdx.x;
+  unsigned long long new_bits;
+  unsigned long long x = diff_word_clamped(old_w, new_w, w, n_words, n_bits, new_bits);
+  // this thread's share of the counts of blocks [0, blockIdx.x): clamped loads, all issued at once
+  constexpr unsigned kCountsPerThread = kDiffDirectBlocks / kDiffThreads;
+  uint32_t v[kCountsPerThread];
+#pragma unroll
+  for (unsigned q = 0; q < kCountsPerThread; ++q) {
+    const unsigned k = threadIdx.x + q * kDiffThreads;
+    v[q] = block_count[k < blockIdx.x ? k : 0u];
+  }
+  uint32_t before = 0;
+#pragma unroll
+  for (unsigned q = 0; q < kCountsPerThread; ++q) before += threadIdx.x + q * kDiffThreads < blockIdx.x ? v[q] : 0u;
   uint32_t offset;
   block_exclusive_scan(before, lds, &offset);
-  const size_t w = static_cast<size_t>(blockIdx.x) * kDiffThreads + threadIdx.x;
-  unsigned long long x = diff_word(old_w, new_w, w, n_words, n_bits);
   uint32_t total;
   uint32_t pos = offset + block_exclusive_scan(__popcll(x), lds, &total);
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_count = offset + total;
@@ -539,7 +598,59 @@ __global__ __launch_bounds__(kDiffThreads) void diff_write_direct_kernel(unsigne
     out[pos++] = static_cast<uint32_t>(w * 64 + b);
     x &= x - 1;
   }
-  if (update_old && w < n_words) old_w[w] = new_w[w];
+  if (update_old && w < n_words) old_w[w] = new_bits;
+}
+
+// The write pass after a cull that stored one change count per mask word
+// (cull_kernel / cull_stream_kernel with Diff): each block of kDiffFusedThreads
+// words adds up the counts of the words before its own, sixteen bytes per
+// thread and load, and then writes as diff_write_direct_kernel.  For at most
+// kDiffFusedWords mask words (2^22 objects): every thread then needs at most
+// kDiffFusedWords / 16 / kDiffFusedThreads = 4 loads of counts, issued at once.
+constexpr unsigned kDiffFusedThreads = 1024;
+constexpr size_t kDiffFusedWords = 65536;
+
+__device__ __forceinline__ uint32_t byte_sum(uint32_t v, uint32_t acc) { return __builtin_amdgcn_sad_u8(v, 0u, acc); }
+
+__global__ __launch_bounds__(kDiffFusedThreads) void diff_write_fused_kernel(unsigned long long* __restrict__ old_w,
+                                                                             const unsigned long long* __restrict__ new_w,
+                                                                             size_t n_words, size_t n_bits,
+                                                                             const uint4* __restrict__ word_count,
+                                                                             uint32_t* __restrict__ out, int update_old,
+                                                                             uint32_t* __restrict__ d_count)
+{
+  __shared__ uint32_t lds[kDiffFusedThreads / kWave + 1];
+  // this thread's words first: their loads are then in flight with those of the counts
+  const size_t w = static_cast<size_t>(blockIdx.x) * kDiffFusedThreads + threadIdx.x;
+  unsigned long long new_bits;
+  unsigned long long x = diff_word_clamped(old_w, new_w, w, n_words, n_bits, new_bits);
+  // this thread's share of the counts of words [0, kDiffFusedThreads * blockIdx.x), clamped loads
+  constexpr unsigned kChunksPerThread = kDiffFusedWords / 16 / kDiffFusedThreads;
+  const size_t chunks = static_cast<size_t>(blockIdx.x) * (kDiffFusedThreads / 16);
+  uint4 c[kChunksPerThread];
+#pragma unroll
+  for (unsigned q = 0; q < kChunksPerThread; ++q) {
+    const size_t k = threadIdx.x + static_cast<size_t>(q) * kDiffFusedThreads;
+    c[q] = word_count[k < chunks ? k : 0];
+  }
+  uint32_t before = 0;
+#pragma unroll
+  for (unsigned q = 0; q < kChunksPerThread; ++q) {
+    const uint32_t part = byte_sum(c[q].w, byte_sum(c[q].z, byte_sum(c[q].y, byte_sum(c[q].x, 0u))));
+    before += threadIdx.x + static_cast<size_t>(q) * kDiffFusedThreads < chunks ? part : 0u;
+  }
+  uint32_t offset;
+  block_exclusive_scan<kDiffFusedThreads>(before, lds, &offset);
+  uint32_t total;
+  uint32_t pos = offset + block_exclusive_scan<kDiffFusedThreads>(__popcll(x), lds, &total);
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) *d_count = offset + total;
+  while (x) {
+    const unsigned b = __ffsll(static_cast<long long>(x)) - 1;
+    if (pos < n_bits) out[pos] = static_cast<uint32_t>(w * 64 + b);  // (always, when the counts match the masks)
+    ++pos;
+    x &= x - 1;
+  }
+  if (update_old && w < n_words) old_w[w] = new_bits;
 }
 
 unsigned grid_for(size_t work, unsigned per_block)
@@ -556,6 +667,61 @@ dpc_cull_config current_cull_config()
   return c;
 }
 
+// Argument checks of the single-frustum cull (what: the caller's name).
+int check_cull_args(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+                    const float* d_planes, size_t ld, const float* vp, const uint64_t* d_mask, const char* what)
+{
+  if (!d_box_a || !d_box_b || !d_planes || !d_mask || !vp)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: null pointer argument", what);
+  if (!aligned(d_box_a, 16) || !aligned(d_planes, 16) || !aligned(d_box_b, 8) || !aligned(d_mask, 8))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: misaligned device pointer", what);
+  if (!d_xform && ld < n_objects)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: %zu objects without indices need ld >= n (ld=%zu)", what,
+                     n_objects, ld);
+  if (ld == 0) return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: no matrices (ld = 0)", what);
+  return DPC_OK;
+}
+
+// The cull's launch in the tuned shape; with Diff, also the per-word change
+// counts against d_old (count: one byte per mask word).
+template <bool Diff>
+int launch_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+                const float* d_planes, size_t ld, const float* vp, uint64_t* d_mask, const uint64_t* d_old,
+                uint8_t* count, void* stream)
+{
+  Mat16 m;
+  for (int k = 0; k < 16; ++k) m.m[k] = vp[k];
+  const auto* a = reinterpret_cast<const float4*>(d_box_a);
+  const auto* b = reinterpret_cast<const float2*>(d_box_b);
+  const auto* p = reinterpret_cast<const float4*>(d_planes);
+  auto* out = reinterpret_cast<unsigned long long*>(d_mask);
+  const auto* old = reinterpret_cast<const unsigned long long*>(d_old);
+  const dpc_cull_config cfg = current_cull_config();
+  const size_t tiles = (n_objects + kWave - 1) / kWave;
+  hipStream_t s = as_stream(stream);
+  if (cfg.waves_per_cu == 0) {
+    if (d_xform)
+      hipLaunchKernelGGL((cull_kernel<true, Diff>), dim3(grid_for(n_objects, kCullThreads)), dim3(kCullThreads), 0, s,
+                         a, b, d_xform, n_objects, p, ld, m, out, old, count);
+    else
+      hipLaunchKernelGGL((cull_kernel<false, Diff>), dim3(grid_for(n_objects, kPlainThreads)), dim3(kPlainThreads), 0,
+                         s, a, b, d_xform, n_objects, p, ld, m, out, old, count);
+  } else {
+    const size_t waves_per_block = kCullThreads / kWave;
+    size_t blocks = static_cast<size_t>(device_cu_count()) * cfg.waves_per_cu / waves_per_block;
+    const size_t needed = (tiles + waves_per_block - 1) / waves_per_block;
+    if (blocks > needed) blocks = needed;
+    if (blocks == 0) blocks = 1;
+    if (d_xform)
+      hipLaunchKernelGGL((cull_stream_kernel<true, Diff>), dim3(static_cast<unsigned>(blocks)), dim3(kCullThreads), 0,
+                         s, a, b, d_xform, n_objects, p, ld, m, out, old, count);
+    else
+      hipLaunchKernelGGL((cull_stream_kernel<false, Diff>), dim3(static_cast<unsigned>(blocks)), dim3(kCullThreads),
+                         0, s, a, b, d_xform, n_objects, p, ld, m, out, old, count);
+  }
+  return check_launch("cull_kernel");
+}
+
 }  // namespace
 }  // namespace dpc
 
@@ -567,43 +733,8 @@ int dpc_cull(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
              const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, void* stream)
 {
   if (n_objects == 0) return DPC_OK;
-  if (!d_box_a || !d_box_b || !d_planes || !d_mask || !vp)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: null pointer argument");
-  if (!aligned(d_box_a, 16) || !aligned(d_planes, 16) || !aligned(d_box_b, 8) || !aligned(d_mask, 8))
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: misaligned device pointer");
-  if (!d_xform && ld < n_objects)
-    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: %zu objects without indices need ld >= n (ld=%zu)",
-                     n_objects, ld);
-  if (ld == 0) return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_cull: no matrices (ld = 0)");
-  Mat16 m;
-  for (int k = 0; k < 16; ++k) m.m[k] = vp[k];
-  const auto* a = reinterpret_cast<const float4*>(d_box_a);
-  const auto* b = reinterpret_cast<const float2*>(d_box_b);
-  const auto* p = reinterpret_cast<const float4*>(d_planes);
-  auto* out = reinterpret_cast<unsigned long long*>(d_mask);
-  const dpc_cull_config cfg = current_cull_config();
-  const size_t tiles = (n_objects + kWave - 1) / kWave;
-  if (cfg.waves_per_cu == 0) {
-    if (d_xform)
-      hipLaunchKernelGGL(cull_kernel<true>, dim3(grid_for(n_objects, kCullThreads)), dim3(kCullThreads), 0,
-                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
-    else
-      hipLaunchKernelGGL(cull_kernel<false>, dim3(grid_for(n_objects, kPlainThreads)), dim3(kPlainThreads), 0,
-                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
-  } else {
-    const size_t waves_per_block = kCullThreads / kWave;
-    size_t blocks = static_cast<size_t>(device_cu_count()) * cfg.waves_per_cu / waves_per_block;
-    const size_t needed = (tiles + waves_per_block - 1) / waves_per_block;
-    if (blocks > needed) blocks = needed;
-    if (blocks == 0) blocks = 1;
-    if (d_xform)
-      hipLaunchKernelGGL(cull_stream_kernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kCullThreads), 0,
-                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
-    else
-      hipLaunchKernelGGL(cull_stream_kernel<false>, dim3(static_cast<unsigned>(blocks)), dim3(kCullThreads), 0,
-                         as_stream(stream), a, b, d_xform, n_objects, p, ld, m, out);
-  }
-  return check_launch("cull_kernel");
+  if (int rc = check_cull_args(d_box_a, d_box_b, d_xform, n_objects, d_planes, ld, vp, d_mask, "dpc_cull")) return rc;
+  return launch_cull<false>(d_box_a, d_box_b, d_xform, n_objects, d_planes, ld, vp, d_mask, nullptr, nullptr, stream);
 }
 
 int dpc_cull_get_config(dpc_cull_config* cfg)
@@ -850,6 +981,44 @@ int dpc_result_diff_noalloc(uint64_t* d_old, const uint64_t* d_new, size_t n_bit
   rc = scratch_of_current_device(&counts, "dpc_result_diff_noalloc");
   if (rc) return rc;
   return launch_result_diff(d_old, d_new, n_bits, d_changed_idx, d_count, update_old, counts, stream);
+}
+
+size_t dpc_cull_diff_workspace(size_t n_objects)
+{
+  const size_t words = (n_objects + 63) / 64;
+  const size_t counts = (words + 15) / 16 * 16;  // one byte per mask word, read sixteen at a time
+  const size_t diff = dpc_result_diff_workspace(n_objects);
+  return counts > diff ? counts : diff;
+}
+
+int dpc_cull_diff(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform, size_t n_objects,
+                  const float* d_planes, size_t ld, const float vp[16], uint64_t* d_mask, uint64_t* d_old,
+                  uint32_t* d_changed_idx, uint32_t* d_count, int update_old, void* d_workspace, void* stream)
+{
+  const char* what = "dpc_cull_diff";
+  int rc = check_result_diff_args(d_old, d_mask, n_objects, d_changed_idx, d_count, what);
+  if (rc) return rc;
+  if (n_objects == 0) return dpc_fill_u32(d_count, 0u, 1, stream);
+  if ((rc = check_cull_args(d_box_a, d_box_b, d_xform, n_objects, d_planes, ld, vp, d_mask, what))) return rc;
+  if (!d_workspace || !aligned(d_workspace, 16))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: null or misaligned workspace", what);
+  if (d_old == d_mask) return set_error(DPC_ERR_INVALID_ARGUMENT, "%s: the old and new masks are one buffer", what);
+  const size_t n_words = (n_objects + 63) / 64;
+  if (n_words > kDiffFusedWords) {  // the prefix sums would grow quadratically: count in a launch of its own
+    rc = launch_cull<false>(d_box_a, d_box_b, d_xform, n_objects, d_planes, ld, vp, d_mask, nullptr, nullptr, stream);
+    if (rc) return rc;
+    return launch_result_diff(d_old, d_mask, n_objects, d_changed_idx, d_count, update_old,
+                              static_cast<uint32_t*>(d_workspace), stream);
+  }
+  auto* counts = static_cast<uint8_t*>(d_workspace);
+  rc = launch_cull<true>(d_box_a, d_box_b, d_xform, n_objects, d_planes, ld, vp, d_mask, d_old, counts, stream);
+  if (rc) return rc;
+  const size_t blocks = (n_words + kDiffFusedThreads - 1) / kDiffFusedThreads;
+  hipLaunchKernelGGL(diff_write_fused_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kDiffFusedThreads), 0,
+                     as_stream(stream), reinterpret_cast<unsigned long long*>(d_old),
+                     reinterpret_cast<const unsigned long long*>(d_mask), n_words, n_objects,
+                     reinterpret_cast<const uint4*>(counts), d_changed_idx, update_old, d_count);
+  return check_launch("diff_write_fused_kernel");
 }
 
 }  // extern "C"

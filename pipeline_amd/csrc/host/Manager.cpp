@@ -291,13 +291,16 @@ class ManagerImpl : public culling::Manager {
     const size_t n = g->m_objects.size();
     const size_t words = (n + 63) / 64;
     g->mask.reserve(std::max<size_t>(words, 1));
-    if (n) {
-      const float* planes = g->m_tree ? g->m_tree->deviceWorldPlanes() : g->planes.p;
-      const size_t ld = g->m_tree ? g->m_tree->deviceLd() : g->m_matricesCount;
-      const uint32_t* xf = g->m_tree ? g->xform_slot.p : g->xform.p;
-      check(dpc_cull(g->box_a.p, g->box_b.p, xf, n, planes, ld, viewProjection.m, g->mask.p, m_stream), "dpc_cull");
-    }
-    updateResult(*g, *r, g->mask.p, n);
+    const float* planes = g->m_tree ? g->m_tree->deviceWorldPlanes() : g->planes.p;
+    const size_t ld = g->m_tree ? g->m_tree->deviceLd() : g->m_matricesCount;
+    const uint32_t* xf = g->m_tree ? g->xform_slot.p : g->xform.p;
+    // the cull and the diff of its mask against the result's bits in one call
+    // (the cull counts each word's changed bits for the diff)
+    updateResult(*g, *r, g->mask.p, n, [&](ResultImpl& res) {
+      check(dpc_cull_diff(g->box_a.p, g->box_b.p, xf, n, planes, ld, viewProjection.m, g->mask.p, res.d_old.p,
+                          res.d_changed.p, res.d_count.p, 1, res.d_ws.p, m_stream),
+            "dpc_cull_diff");
+    });
   }
 
   Box3f getBoundingBox(GroupSharedPtr const& group) const override
@@ -495,6 +498,17 @@ class ManagerImpl : public culling::Manager {
   // ResultBitSet::updateChanged (ResultBitSet.cpp:61-108) with the diff on the device.
   void updateResult(GroupImpl& g, ResultImpl& r, const uint64_t* d_new, size_t n)
   {
+    updateResult(g, r, d_new, n, [&](ResultImpl& res) {
+      check(dpc_result_diff(res.d_old.p, d_new, n, res.d_changed.p, res.d_count.p, 1, res.d_ws.p, m_stream),
+            "dpc_result_diff");
+    });
+  }
+
+  // The same with the step that leaves the new mask in d_new and the diff's
+  // outputs in the result's buffers given by the caller (diff).
+  template <class Diff>
+  void updateResult(GroupImpl& g, ResultImpl& r, const uint64_t* d_new, size_t n, Diff&& diff)
+  {
     TraceRange trace("ResultBitSet::updateChanged");
     if (r.m_incarnation != g.m_incarnation) {
       r.m_incarnation = g.m_incarnation;
@@ -507,8 +521,8 @@ class ManagerImpl : public culling::Manager {
     r.d_old.reserve(std::max<size_t>(words, 1));
     r.d_changed.reserve(std::max<size_t>(n, 1));
     r.d_count.reserve(1);
-    const size_t ws = dpc_result_diff_workspace(n);
-    r.d_ws.reserve(std::max<size_t>(ws, 4));
+    const size_t ws = dpc_cull_diff_workspace(n);  // (at least dpc_result_diff_workspace(n))
+    r.d_ws.reserve(std::max<size_t>(ws, 16));
     // The result changes only when every step succeeded: after a failure the
     // host bits are still the last successful cull's, and the device copy,
     // which the diff may already have overwritten, is uploaded again next time.
@@ -519,8 +533,7 @@ class ManagerImpl : public culling::Manager {
         check(dpc_memcpy_h2d(r.d_old.p, r.m_bits.data(), words * 8, m_stream), "upload result bits");
         r.m_deviceStale = false;
       }
-      check(dpc_result_diff(r.d_old.p, d_new, n, r.d_changed.p, r.d_count.p, 1, r.d_ws.p, m_stream),
-            "dpc_result_diff");
+      diff(r);
       uint32_t count = 0;
       check(dpc_memcpy_d2h(&count, r.d_count.p, 4, m_stream), "download count");
       if (words) check(dpc_memcpy_d2h(bits.data(), d_new, words * 8, m_stream), "download result bits");

@@ -40,3 +40,17 @@ def _announce_gpu_case(request):
         sys.stdout.write(f"\n[gpu case] {request.node.nodeid}\n")
         sys.stdout.flush()
     yield
+
+
+@pytest.fixture(params=[0, 16, 3], ids=["grid", "stream16", "stream3"])
+def cull_shape(request):
+    """Runs a test under both launch shapes of dpc_cull (full grid; persistent
+    streaming grid, also with few waves so every wave walks many tiles)."""
+    import ctypes as C
+    from pipeline_amd import _lib
+    lib = _lib.load()
+    old = _lib.CullConfig()
+    _lib.check(lib.dpc_cull_get_config(C.byref(old)))
+    _lib.check(lib.dpc_cull_set_config(C.byref(_lib.CullConfig(request.param))))
+    yield request.param
+    _lib.check(lib.dpc_cull_set_config(C.byref(old)))

@@ -51,6 +51,15 @@ def test_argument_errors_are_reported_before_any_device_call(lib):
     # dpc_result_diff_noalloc: more bits than its fixed scratch serves
     assert lib.dpc_result_diff_noalloc(a, a, (1 << 30) + 64, a, a, 0, None) == inval
     assert b"exceed" in lib.dpc_last_error()
+    # dpc_cull_diff: no workspace, a misaligned one, the old mask as the new one
+    import ctypes
+    vp = (ctypes.c_float * 16)()
+    assert lib.dpc_cull_diff(a, a, None, 100, a, 100, vp, a, a + 4096, a, a, 1, None, None) == inval
+    assert b"workspace" in lib.dpc_last_error()
+    assert lib.dpc_cull_diff(a, a, None, 100, a, 100, vp, a, a + 4096, a, a, 1, a + 8, None) == inval
+    assert lib.dpc_cull_diff(a, a, None, 100, a, 100, vp, a, a, a, a, 1, a, None) == inval
+    assert b"one buffer" in lib.dpc_last_error()
+    assert lib.dpc_cull_diff_workspace(1 << 22) >= (1 << 16)
     # dpc_inverse_transpose: more matrices than the leading dimension holds
     assert lib.dpc_inverse_transpose(a, 5, None, 10, a, 10, None, None) == inval
     assert b"exceeds" in lib.dpc_last_error()

@@ -12,20 +12,6 @@ F = np.float32
 ID = np.eye(4, dtype=F)
 
 
-@pytest.fixture(params=[0, 16, 3], ids=["grid", "stream16", "stream3"])
-def cull_shape(request):
-    """Runs a test under both launch shapes of dpc_cull (full grid; persistent
-    streaming grid, also with few waves so every wave walks many tiles)."""
-    import ctypes as C
-    from pipeline_amd import _lib
-    lib = _lib.load()
-    old = _lib.CullConfig()
-    _lib.check(lib.dpc_cull_get_config(C.byref(old)))
-    _lib.check(lib.dpc_cull_set_config(C.byref(_lib.CullConfig(request.param))))
-    yield request.param
-    _lib.check(lib.dpc_cull_set_config(C.byref(old)))
-
-
 def oracle_mask(orc, lower, upper, matrices, vp, xform=None):
     l4, e4 = scenes.extents(lower, upper)
     x = np.arange(lower.shape[0]) if xform is None else xform
