@@ -316,6 +316,28 @@ __device__ __forceinline__ Box6 box6_of(const Box6i& b)
   return r;
 }
 
+// One object's inputs to the bounding box: its box and its matrix's rows (row
+// index clamped to 0 when it lies outside the array; such an object is skipped).
+struct BoxIn {
+  float4 a;
+  float2 b;
+  float4 r0, r1, r2, r3;
+};
+
+__device__ __forceinline__ BoxIn load_box_in(const float4* __restrict__ box_a, const float2* __restrict__ box_b,
+                                             const float4* __restrict__ planes, size_t ld, size_t i, size_t m)
+{
+  const size_t c = m < ld ? m : 0;
+  BoxIn d;
+  d.a = box_a[i];
+  d.b = box_b[i];
+  d.r0 = planes[c];
+  d.r1 = planes[ld + c];
+  d.r2 = planes[2 * ld + c];
+  d.r3 = planes[3 * ld + c];
+  return d;
+}
+
 // Each wave scans one contiguous chunk of objects, lane l taking objects
 // begin + l, begin + l + 64, ... (coalesced loads).  A lane first takes the box
 // of its object's eight corners, then folds it into its running box with the
@@ -323,24 +345,39 @@ __device__ __forceinline__ Box6 box6_of(const Box6i& b)
 // workgroup's waves hold consecutive chunks, so folding their boxes in wave
 // order with strict comparisons keeps the first index among equal extrema; one
 // partial per workgroup is stored, in index order of the chunks.
+template <bool Indexed>
 __global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4* __restrict__ box_a,
                                                                    const float2* __restrict__ box_b,
                                                                    const uint32_t* __restrict__ xform, size_t n,
                                                                    const float4* __restrict__ planes, size_t ld,
                                                                    size_t chunk, Box6* __restrict__ partial)
 {
-  const size_t wave = static_cast<size_t>(blockIdx.x) * kBoxWavesPerBlock + (threadIdx.x / kWave);
+  // wave-uniform in scalar registers, so the loop below branches on scalars
+  const size_t wave = __builtin_amdgcn_readfirstlane(blockIdx.x * kBoxWavesPerBlock + threadIdx.x / kWave);
   const unsigned lane = threadIdx.x & (kWave - 1);
   const size_t begin = wave * chunk;
   size_t end = begin + chunk;
   if (end > n) end = n;
   Box6i acc = box6i_empty();
-  for (size_t i = begin + lane; i < end; i += kWave) {
-    const size_t m = xform ? static_cast<size_t>(xform[i]) : i;
-    if (m >= ld) continue;  // (an object whose matrix index is outside the array adds nothing)
-    const float4 r0 = planes[m], r1 = planes[ld + m], r2 = planes[2 * ld + m], r3 = planes[3 * ld + m];
+  // Round k takes object begin + lane + 64k.  The loads are clamped and
+  // unconditional and run ahead of the folds, in two register sets that take
+  // turns (no copies, which would wait for the loads): the box and matrix rows
+  // of round k + 1 and the matrix index of round k + 2 are in flight while round
+  // k is folded.
+  const size_t count = end > begin ? end - begin : 0;
+  const size_t rounds = (count + kWave - 1) / kWave;  // the same on every lane of the wave
+  auto object_of = [&](size_t k) -> size_t {
+    const size_t i = begin + lane + k * kWave;
+    return i < end ? i : begin;  // (begin < end whenever rounds > 0)
+  };
+  auto matrix_of = [&](size_t i) -> size_t { return Indexed ? static_cast<size_t>(xform[i]) : i; };
+  // Folds round k's object (index i, matrix m, inputs d) into the lane's box.
+  auto fold = [&](const BoxIn& d, size_t i, size_t m, size_t k) {
+    // a lane past the chunk's end, or an object whose matrix index is outside
+    // the array, adds nothing
+    const bool valid = (begin + lane + k * kWave < end) & (m < ld);
     // ManagerBitSet.cpp:282-294: v0 = lower * M, x/y/z = extent[k] * row k, same add tree as the cull.
-    const Obb o = make_obb(box_a[i], box_b[i], r0, r1, r2, r3);
+    const Obb o = make_obb(d.a, d.b, d.r0, d.r1, d.r2, d.r3);
     const float4 v1 = add4(o.p, o.ex);
     const float4 v2 = add4(o.p, o.ey);
     const float4 v3 = add4(v1, o.ey);
@@ -359,14 +396,38 @@ __global__ __launch_bounds__(kBoxThreads) void bbox_partial_kernel(const float4*
     box_take(mine, v7);
     const uint32_t idx = static_cast<uint32_t>(i - begin);
     for (int c = 0; c < 3; ++c) {
-      if (acc.lo[c] > mine.lo[c]) {
-        acc.lo[c] = mine.lo[c];
-        acc.ilo[c] = idx;
-      }
-      if (acc.hi[c] < mine.hi[c]) {
-        acc.hi[c] = mine.hi[c];
-        acc.ihi[c] = idx;
-      }
+      const bool lower = valid & (acc.lo[c] > mine.lo[c]);
+      acc.lo[c] = lower ? mine.lo[c] : acc.lo[c];
+      acc.ilo[c] = lower ? idx : acc.ilo[c];
+      const bool higher = valid & (acc.hi[c] < mine.hi[c]);
+      acc.hi[c] = higher ? mine.hi[c] : acc.hi[c];
+      acc.ihi[c] = higher ? idx : acc.ihi[c];
+    }
+  };
+  if (rounds) {
+    size_t ia = object_of(0), ma = matrix_of(ia);
+    BoxIn a = load_box_in(box_a, box_b, planes, ld, ia, ma);
+    size_t in = object_of(1), mn = matrix_of(in);  // the next round's object and matrix index
+    BoxIn b;
+    size_t ib = 0, mb = 0;
+    for (size_t k = 0; k < rounds; k += 2) {
+      // round k is in a; load round k + 1 into b and the index of round k + 2
+      ib = in;
+      mb = mn;
+      in = object_of(k + 2);
+      mn = matrix_of(in);
+      b = load_box_in(box_a, box_b, planes, ld, ib, mb);
+      __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the fold
+      fold(a, ia, ma, k);
+      if (k + 1 >= rounds) break;
+      // round k + 1 is in b; load round k + 2 into a and the index of round k + 3
+      ia = in;
+      ma = mn;
+      in = object_of(k + 3);
+      mn = matrix_of(in);
+      a = load_box_in(box_a, box_b, planes, ld, ia, ma);
+      __builtin_amdgcn_sched_barrier(0);
+      fold(b, ib, mb, k + 1);
     }
   }
   __shared__ Box6 waves[kBoxWavesPerBlock];
@@ -850,10 +911,15 @@ int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
   chunk = (chunk + kWave - 1) / kWave * kWave;
   if (chunk == 0) chunk = kWave;
   auto* partial = static_cast<Box6*>(d_workspace);
-  hipLaunchKernelGGL(bbox_partial_kernel, dim3(static_cast<unsigned>(waves / kBoxWavesPerBlock)), dim3(kBoxThreads), 0,
-                     as_stream(stream), reinterpret_cast<const float4*>(d_box_a),
-                     reinterpret_cast<const float2*>(d_box_b), d_xform, n_objects,
-                     reinterpret_cast<const float4*>(d_planes), ld, chunk, partial);
+  const dim3 grid(static_cast<unsigned>(waves / kBoxWavesPerBlock));
+  if (d_xform)
+    hipLaunchKernelGGL(bbox_partial_kernel<true>, grid, dim3(kBoxThreads), 0, as_stream(stream),
+                       reinterpret_cast<const float4*>(d_box_a), reinterpret_cast<const float2*>(d_box_b), d_xform,
+                       n_objects, reinterpret_cast<const float4*>(d_planes), ld, chunk, partial);
+  else
+    hipLaunchKernelGGL(bbox_partial_kernel<false>, grid, dim3(kBoxThreads), 0, as_stream(stream),
+                       reinterpret_cast<const float4*>(d_box_a), reinterpret_cast<const float2*>(d_box_b), d_xform,
+                       n_objects, reinterpret_cast<const float4*>(d_planes), ld, chunk, partial);
   int rc = check_launch("bbox_partial_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(bbox_final_kernel, dim3(1), dim3(kBoxFinalThreads), 0, as_stream(stream), partial,
