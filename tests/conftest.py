@@ -34,8 +34,8 @@ def lib():
 
 @pytest.fixture(autouse=True)
 def _announce_gpu_case(request):
-    # Print the case name before each GPU test runs, so a run that ends the box
-    # still shows which case was running.
+    # Print the case name before each GPU test runs, so a run that takes the
+    # machine down still shows which case was running.
     if request.node.get_closest_marker("gpu") is not None:
         sys.stdout.write(f"\n[gpu case] {request.node.nodeid}\n")
         sys.stdout.flush()
