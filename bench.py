@@ -846,7 +846,8 @@ def main():
         # Bit parity with the reference forbids fused multiply-add (it rounds add and
         # mul separately) and this pool forbids packed FP32, so one lane does one
         # unfused op per clock: the issue ceiling below is a quarter of the published
-        # peak, and it counts the 48 compares per test as the VALU operations they are.
+        # peak, and it counts the 48 compares per test as the VALU operations they are
+        # (compares issue faster than adds and muls, so the kernel can reach 1.0 of it).
         "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": measured_traffic("cull_batch_kernel"),
                      "traffic_unit": "bytes per launch beyond L2 (FETCH_SIZE + WRITE_SIZE, copy-calibrated; "
@@ -857,7 +858,10 @@ def main():
                      "issue_ceiling": {"achieved": achieved_tops, "peak": ISSUE_CEILING_TOPS, "unit": "T VALU op/s",
                                        "frac": achieved_tops / ISSUE_CEILING_TOPS,
                                        "basis": f"{OPS_PER_OBJECT} ops/object + {OPS_PER_TEST} ops/test (add, mul "
-                                                "and compare); 256 CU x 64 lanes x 2.4 GHz, no FMA, no packed FP32"},
+                                                "and compare); 256 CU x 64 lanes x 2.4 GHz, no FMA, no packed FP32. "
+                                                "Independent FP32 adds or muls alone issue at 38.5-38.8 T lane-op/s "
+                                                "here; compares issue faster, so about 1.0 is the issue limit of this "
+                                                "instruction mix (profiles/cull_batch_kernel_valu.md)"},
                      "hbm": {"achieved": hbm_bytes / (k_mean_b * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": hbm_bytes / (k_mean_b * 1e-3) / 1e9 / PEAK_HBM_GBS,
                              "basis": f"{BYTES_PER_OBJECT} B per referenced object + 1 bit per test"}},

@@ -86,6 +86,43 @@ __device__ __forceinline__ void load_item(const SceneDev* __restrict__ scenes, c
 // and the next frustum matrix is loaded (scalar memory) while the current one is
 // in use.  Both prefetches are unconditional (indices clamped to the last item
 // or frustum), so no wait lands in front of the FP32 work.
+// Tests a work item's objects (slot, o) against its frusta and stores one mask
+// word per frustum and 64 objects.  The frusta take turns in two sets of scalar
+// registers: while one frustum is tested, the next one's matrix is loaded into
+// the other set (no register copies between frusta).  Every lane stores the
+// wave's mask word (the same value to the same address): a store from lane 0
+// alone would change the exec mask around it, scalar instructions in a loop
+// whose scalar work is a bound.
+template <int K>
+__device__ __forceinline__ void test_frusta(const WorkItem& item, const Slot (&slot)[K], const Obb (&o)[K],
+                                            const KPair* __restrict__ pairs, const float* __restrict__ vps,
+                                            unsigned long long* __restrict__ masks)
+{
+  auto test = [&](const KPair& q, const Mat16& m) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const bool vis = slot_visible(slot[k], obb_visible(o[k], m.m));
+      const unsigned long long bits = __ballot(vis);
+      // (a work item holds at least one object: the first word is always stored)
+      if (k == 0 || static_cast<uint32_t>(kWave * k) < item.count) masks[q.mask_word + item.tile_word + k] = bits;
+    }
+  };
+  const uint32_t last = item.pair_count ? item.pair_count - 1 : 0;  // (no item is made without pairs)
+  auto pair_at = [&](uint32_t p) { return pairs[item.pair_begin + (p < last ? p : last)]; };
+  KPair pa = pair_at(0), pb;
+  Mat16 va, vb;
+  load_vp(vps, pa.frustum, va);
+  for (uint32_t p = 0; p < item.pair_count; p += 2) {
+    pb = pair_at(p + 1);
+    load_vp(vps, pb.frustum, vb);
+    test(pa, va);
+    if (p + 1 >= item.pair_count) break;
+    pa = pair_at(p + 2);
+    load_vp(vps, pa.frustum, va);
+    test(pb, vb);
+  }
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void cull_batch_kernel(const WorkItem* __restrict__ items, uint32_t n_items,
                                                          const KPair* __restrict__ pairs,
@@ -111,23 +148,7 @@ __global__ __launch_bounds__(256) void cull_batch_kernel(const WorkItem* __restr
     Obb o[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) o[k] = obb_of(raw[k]);
-    KPair pr = pairs[item.pair_begin];
-    Mat16 vp;
-    load_vp(vps, pr.frustum, vp);
-    for (uint32_t p = 0; p < item.pair_count; ++p) {
-      const uint32_t pn = p + 1 < item.pair_count ? p + 1 : p;
-      const KPair next_pr = pairs[item.pair_begin + pn];
-      Mat16 next_vp;
-      load_vp(vps, next_pr.frustum, next_vp);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const bool vis = slot_visible(slot[k], obb_visible(o[k], vp.m));
-        const unsigned long long bits = __ballot(vis);
-        if (lane == 0 && static_cast<uint32_t>(kWave * k) < item.count) masks[pr.mask_word + item.tile_word + k] = bits;
-      }
-      pr = next_pr;
-      vp = next_vp;
-    }
+    test_frusta<K>(item, slot, o, pairs, vps, masks);
     if (nit >= n_items) break;
     it = nit;
     item = next_item;
@@ -207,23 +228,7 @@ __global__ __launch_bounds__(256) void cull_serve_kernel(const ServeEntry* __res
     Obb o[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) o[k] = obb_of(raw[k]);
-    KPair pr = pairs[item.pair_begin];
-    Mat16 vp;
-    load_vp(vps, pr.frustum, vp);
-    for (uint32_t p = 0; p < item.pair_count; ++p) {
-      const uint32_t pn = p + 1 < item.pair_count ? p + 1 : p;
-      const KPair next_pr = pairs[item.pair_begin + pn];
-      Mat16 next_vp;
-      load_vp(vps, next_pr.frustum, next_vp);
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const bool vis = slot_visible(slot[k], obb_visible(o[k], vp.m));
-        const unsigned long long bits = __ballot(vis);
-        if (lane == 0 && static_cast<uint32_t>(kWave * k) < item.count) masks[pr.mask_word + item.tile_word + k] = bits;
-      }
-      pr = next_pr;
-      vp = next_vp;
-    }
+    test_frusta<K>(item, slot, o, pairs, vps, masks);
     if (!more) break;
     e = ne;
     a = na;
