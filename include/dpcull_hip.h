@@ -309,7 +309,9 @@ int dpc_bbox(const float* d_box_a, const float* d_box_b, const uint32_t* d_xform
  * batch, on one stream while any of them is in flight; requests are admitted
  * while the device memory they need fits the budget left after the resident
  * scenes (a quarter of the budget, at most 1 GiB, is the in-flight arena,
- * split over streams x batches_per_stream batches). */
+ * split over streams x batches_per_stream batches).  Each batch also has a
+ * pinned host arena of the same size, at most 32 MiB, where its descriptors and
+ * copied deltas are gathered for one transfer. */
 typedef struct dpc_server dpc_server;
 
 typedef struct dpc_server_config {
@@ -368,7 +370,10 @@ int dpc_server_remove_scene(dpc_server* server, uint32_t scene_id);
  * are borrowed until dpc_server_wait returns for the request.  With
  * direct_delta_reads, delta arrays that each lie within one pinned host
  * allocation (dpc_host_alloc) are read by the device directly, with no copy;
- * others are copied first (a DMA transfer from pinned memory).  A masks_out
+ * others are copied first: gathered by the host with the batch's other deltas
+ * into the batch's pinned arena and transferred in one DMA copy (matrix arrays
+ * above 32 MiB, or batches whose deltas exceed the arena, are transferred
+ * from the caller's arrays instead).  A masks_out
  * whose mask_capacity_words lie within one pinned host allocation is written by
  * the device directly (one copy-out launch per batch, no DMA transfer); any
  * other masks_out receives a copy. */

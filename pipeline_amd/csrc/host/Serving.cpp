@@ -48,6 +48,11 @@ constexpr uint32_t kMaxBatch = 64;
 constexpr uint32_t kMaxFrusta = 64;
 // the pinned staging area of one batch's descriptors (entries, pairs, frusta, copy-outs)
 constexpr uint64_t kHostArena = 4096 + kMaxBatch * 32 + kMaxBatch * kMaxFrusta * (16 + 64) + kMaxBatch * 32 + 4 * 256;
+// ... which also stages the batch's deltas when they fit: the batch then
+// crosses PCIe in one copy instead of two per request
+constexpr uint64_t kHostStageMax = 32ull << 20;
+// a delta's matrices up to this size are staged; a larger array is copied on its own
+constexpr uint64_t kStageMatsMax = kHostStageMax;
 
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
@@ -92,8 +97,9 @@ struct StreamSlot {
   int stream_index = 0;
   hipEvent_t done = nullptr;
   char* d_arena = nullptr;
-  char* h_arena = nullptr;  // pinned, kHostArena bytes
+  char* h_arena = nullptr;  // pinned, h_cap bytes: the device arena's first h_cap bytes staged on the host
   uint64_t cap = 0;
+  uint64_t h_cap = 0;
   bool busy = false;     // formed and not yet completed
   bool issued = false;   // its issuer has recorded the completion event
   std::vector<std::shared_ptr<Request>> reqs;
@@ -256,14 +262,28 @@ void dpc_server::issue_batch(StreamSlot& slot)
   const uint64_t copies_off = off;
   off = align_up(off + sizeof(dpc::CopyOut) * pick.size(), 256);
   const uint64_t desc_end = off;
+  // The copied deltas: every index array and the matrices of deltas up to
+  // kStageMatsMax bytes first (the part staged on the host, below), then the
+  // larger matrix arrays, each copied on its own.
   std::vector<uint64_t> idx_off(pick.size()), mat_off(pick.size());
   for (size_t k = 0; k < pick.size(); ++k) {
     const Request& r = *pick[k];
-    const uint64_t staged = r.didx_dev ? 0 : r.n_delta;
+    const uint64_t n = r.didx_dev ? 0 : r.n_delta;
     idx_off[k] = off;
-    off = align_up(off + 4ull * staged, 256);
-    mat_off[k] = off;
-    off = align_up(off + 64ull * staged, 256);
+    off = align_up(off + 4ull * n, 256);
+    if (64ull * n <= kStageMatsMax) {
+      mat_off[k] = off;
+      off = align_up(off + 64ull * n, 256);
+    }
+  }
+  const uint64_t stage_end = off;
+  for (size_t k = 0; k < pick.size(); ++k) {
+    const Request& r = *pick[k];
+    const uint64_t n = r.didx_dev ? 0 : r.n_delta;
+    if (64ull * n > kStageMatsMax) {
+      mat_off[k] = off;
+      off = align_up(off + 64ull * n, 256);
+    }
   }
   const uint64_t masks_off = off;
   auto* entries = reinterpret_cast<dpc::ServeEntry*>(slot.h_arena + entries_off);
@@ -306,7 +326,25 @@ void dpc_server::issue_batch(StreamSlot& slot)
   double t_part[6];
   double copy_us = 0.0;
   t_part[0] = now();
-  // deltas: copy to the device and scatter into the scenes' matrices
+  // The descriptors, the index arrays and the smaller matrix arrays staged
+  // together in the pinned host arena when they fit: one copy for them, then
+  // the larger matrix arrays on their own, then the scatters.  Otherwise each
+  // delta is copied on its own, before its scatter.
+  const bool staged = stage_end <= slot.h_cap;
+  if (staged) {
+    for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
+      const Request& r = *pick[k];
+      if (!r.n_delta || r.didx_dev) continue;
+      std::memcpy(slot.h_arena + idx_off[k], r.didx, 4ull * r.n_delta);
+      if (64ull * r.n_delta <= kStageMatsMax) std::memcpy(slot.h_arena + mat_off[k], r.dmats, 64ull * r.n_delta);
+    }
+    if (rc == DPC_OK) {
+      e = hipMemcpyAsync(slot.d_arena, slot.h_arena, stage_end, hipMemcpyHostToDevice, s);
+      if (e != hipSuccess) rc = hip_fail(e, "batch upload");
+    }
+    copy_us = now() - t_part[0];
+  }
+  // deltas: copy to the device (unless staged above) and scatter into the scenes' matrices
   for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
     const Request& r = *pick[k];
     if (!r.n_delta) continue;
@@ -314,14 +352,16 @@ void dpc_server::issue_batch(StreamSlot& slot)
       rc = dpc::launch_scatter_host(snapshot[k].planes, snapshot[k].n_mat, r.didx_dev, r.dmats_dev, r.n_delta, s);
       continue;
     }
-    const double tc = now();
-    e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(slot.d_arena + mat_off[k], r.dmats, 64ull * r.n_delta, hipMemcpyHostToDevice, s);
-    copy_us += now() - tc;
-    if (e != hipSuccess) {
-      rc = hip_fail(e, "delta upload");
-      break;
+    if (!staged || 64ull * r.n_delta > kStageMatsMax) {
+      const double tc = now();
+      if (!staged) e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(slot.d_arena + mat_off[k], r.dmats, 64ull * r.n_delta, hipMemcpyHostToDevice, s);
+      copy_us += now() - tc;
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "delta upload");
+        break;
+      }
     }
     rc = dpc_matrices_scatter(snapshot[k].planes, snapshot[k].n_mat,
                               reinterpret_cast<const uint32_t*>(slot.d_arena + idx_off[k]),
@@ -329,7 +369,7 @@ void dpc_server::issue_batch(StreamSlot& slot)
   }
   t_part[2] = now();
   t_part[1] = t_part[0] + copy_us;  // parts 0 and 1: copies, then the scatter launches (interleaved in time)
-  if (rc == DPC_OK) {
+  if (rc == DPC_OK && !staged) {
     e = hipMemcpyAsync(slot.d_arena, slot.h_arena, desc_end, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) rc = hip_fail(e, "descriptor upload");
   }
@@ -514,7 +554,8 @@ int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
     sl.stream = srv->streams[sl.stream_index];
     if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&sl.d_arena), per);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_arena), kHostArena, hipHostMallocDefault);
+    sl.h_cap = std::max(kHostArena, std::min(per, kHostStageMax));
+    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_arena), sl.h_cap, hipHostMallocDefault);
     sl.cap = per;
   }
   if (e != hipSuccess) {

@@ -112,6 +112,45 @@ def test_serve_mix_equals_alone_and_oracle(orc):
             assert np.array_equal(m, mix[i][1]), f"scene {s} request {i}: alone != mix"
 
 
+def test_serve_large_deltas(orc):
+    """Copied deltas are staged with the batch's descriptors in the server's pinned
+    host arena (32 MiB per batch) and cross PCIe in one copy.  This covers what does
+    not fit: a delta whose matrices alone exceed the arena (copied on its own
+    after the staged part), and batches of deltas that fit one by one but not
+    together (every delta then copied on its own).  Masks are checked against
+    the oracle replaying every delta in submission order."""
+    scenes = S.serving_scenes(seed=21, n_scenes=3, size_range=(700_000, 700_001))
+    rng = np.random.default_rng(22)
+    sizes = [600_000, 300_000, 300_000, 300_000, 40_000]  # 38 MB of matrices, then 19 MB each, then a small one
+    reqs = []
+    for i, k in enumerate(sizes):
+        s = [0, 0, 1, 2, 1][i]
+        sc = scenes[s]
+        vps = S.serving_requests([sc], seed=100 + i, n_requests=1, rate_per_s=0.0, frusta=(1, 2))[0].vps
+        idx = rng.choice(sc.n, k, replace=False).astype(np.uint32)
+        mats = S.trs_matrices(rng, k)
+        reqs.append(S.ServingRequest(s, vps, idx, mats, 0.0))
+    out = []
+    with Server(max_scenes=4, max_batch_requests=3, streams=1, hbm_budget_bytes=8 << 30) as srv:
+        ids = [srv.add_scene(sc.lower, sc.upper, sc.matrices) for sc in scenes]
+        pending = []
+        for r in reqs:
+            m = np.zeros(srv.mask_words(ids[r.scene], r.vps.shape[0]), np.uint64)
+            pending.append((srv.submit(ids[r.scene], r.vps, m, r.delta_idx, r.delta_mats), m))
+        for rid, m in pending:
+            out.append((srv.wait(rid, 120_000.0), m))
+    assert all(info.status == 0 for info, _ in out)
+    print("batches:", [(info.batch, info.batch_requests) for info, _ in out])
+    rep = Replay(orc, scenes)
+    for i, (r, (info, m)) in enumerate(zip(reqs, out)):
+        want = rep.apply_and_cull(r)
+        n = scenes[r.scene].n
+        words = (n + 63) // 64
+        bad = sum(mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
+                  for f in range(r.vps.shape[0]))
+        assert bad == 0, f"request {i} ({r.delta_idx.size} matrices): {bad} bits differ from the oracle"
+
+
 @pytest.mark.parametrize("direct_reads", [True, False], ids=["direct_reads", "copied"])
 def test_serve_pinned_deltas_read_directly(orc, direct_reads):
     """With direct_delta_reads, deltas whose two arrays lie in pinned host
