@@ -56,6 +56,39 @@ constexpr uint64_t kStageMatsMax = kHostStageMax;
 
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
+// One host copy of a gather: bytes from src to dst.
+struct Piece {
+  char* dst;
+  const char* src;
+  uint64_t bytes;
+};
+
+// Copies every piece.  From 2 MiB on, the bytes are split into equal runs for
+// up to four threads (this one and up to three started here and joined before
+// returning): one thread's memcpy would hold up the batch's transfer.
+void gather(const std::vector<Piece>& pieces)
+{
+  uint64_t total = 0;
+  for (const Piece& p : pieces) total += p.bytes;
+  constexpr uint64_t kSplitMin = 2ull << 20, kPerThreadMin = 1ull << 20;
+  const unsigned parts = total < kSplitMin ? 1u : static_cast<unsigned>(std::min<uint64_t>(4, total / kPerThreadMin));
+  // part t copies the bytes [t * total / parts, (t + 1) * total / parts) of the concatenated pieces
+  auto run = [&](unsigned t) {
+    const uint64_t lo = total * t / parts, hi = total * (t + 1) / parts;
+    uint64_t at = 0;
+    for (const Piece& p : pieces) {
+      const uint64_t a = std::max(lo, at), b = std::min(hi, at + p.bytes);
+      if (a < b) std::memcpy(p.dst + (a - at), p.src + (a - at), b - a);
+      at += p.bytes;
+      if (at >= hi) break;
+    }
+  };
+  std::vector<std::thread> helpers;
+  for (unsigned t = 1; t < parts; ++t) helpers.emplace_back(run, t);
+  run(0);
+  for (auto& h : helpers) h.join();
+}
+
 struct Scene {
   bool live = false;
   bool ready = false;     // uploaded: may be scheduled
@@ -332,12 +365,15 @@ void dpc_server::issue_batch(StreamSlot& slot)
   // delta is copied on its own, before its scatter.
   const bool staged = stage_end <= slot.h_cap;
   if (staged) {
-    for (size_t k = 0; k < pick.size() && rc == DPC_OK; ++k) {
+    std::vector<Piece> pieces;
+    for (size_t k = 0; k < pick.size(); ++k) {
       const Request& r = *pick[k];
       if (!r.n_delta || r.didx_dev) continue;
-      std::memcpy(slot.h_arena + idx_off[k], r.didx, 4ull * r.n_delta);
-      if (64ull * r.n_delta <= kStageMatsMax) std::memcpy(slot.h_arena + mat_off[k], r.dmats, 64ull * r.n_delta);
+      pieces.push_back(Piece{slot.h_arena + idx_off[k], reinterpret_cast<const char*>(r.didx), 4ull * r.n_delta});
+      if (64ull * r.n_delta <= kStageMatsMax)
+        pieces.push_back(Piece{slot.h_arena + mat_off[k], reinterpret_cast<const char*>(r.dmats), 64ull * r.n_delta});
     }
+    gather(pieces);
     if (rc == DPC_OK) {
       e = hipMemcpyAsync(slot.d_arena, slot.h_arena, stage_end, hipMemcpyHostToDevice, s);
       if (e != hipSuccess) rc = hip_fail(e, "batch upload");

@@ -102,7 +102,10 @@ static std::vector<Req> make_requests(std::mt19937& rng, const std::vector<Scene
       vp[12] = 0.3f * u(rng);
       vp[13] = 0.3f * u(rng);
     }
-    r.n_delta = (k % 4 == 3) ? 0 : static_cast<uint32_t>(1 + rng() % (sc.n / 20));
+    // every tenth request rewrites all of its scene's matrices (up to ~3 MB, which
+    // the server gathers on several threads)
+    const bool whole = k % 10 == 5;
+    r.n_delta = (k % 4 == 3) ? 0 : whole ? static_cast<uint32_t>(sc.n) : static_cast<uint32_t>(1 + rng() % (sc.n / 20));
     r.pinned = (k % 3) != 0;
     if (r.n_delta) {
       void *pi = nullptr, *pm = nullptr;
@@ -117,10 +120,11 @@ static std::vector<Req> make_requests(std::mt19937& rng, const std::vector<Scene
       }
       r.idx = static_cast<uint32_t*>(pi);
       r.mats = static_cast<float*>(pm);
-      // distinct indices: start + 7d (mod n) for d < n / 20 never meets itself again
+      // distinct indices: d itself for a whole-scene delta; start + 7d (mod n) for
+      // d < n / 20 never meets itself again
       const uint64_t start = rng() % sc.n;
       for (uint32_t d = 0; d < r.n_delta; ++d) {
-        r.idx[d] = static_cast<uint32_t>((start + 7ull * d) % sc.n);
+        r.idx[d] = whole ? d : static_cast<uint32_t>((start + 7ull * d) % sc.n);
         float* m = &r.mats[16ull * d];
         identity_into(m);
         for (int c = 0; c < 3; ++c) m[12 + c] = 20.f * u(rng);
