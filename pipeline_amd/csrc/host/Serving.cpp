@@ -83,8 +83,15 @@ void gather(const std::vector<Piece>& pieces)
       if (at >= hi) break;
     }
   };
+  // (a helper that cannot be started leaves its part to this thread)
   std::vector<std::thread> helpers;
-  for (unsigned t = 1; t < parts; ++t) helpers.emplace_back(run, t);
+  unsigned started = 1;
+  try {
+    helpers.reserve(parts - 1);
+    for (; started < parts; ++started) helpers.emplace_back(run, started);
+  } catch (...) {
+  }
+  for (unsigned t = started; t < parts; ++t) run(t);
   run(0);
   for (auto& h : helpers) h.join();
 }
