@@ -57,14 +57,19 @@ def test_xbar_integration_frames():
 
 
 def test_host_code_under_sanitizers():
-    """The tile planner (tree_plan.hpp) and the tree bookkeeping (TreeLayout in
-    host/Tree.cpp) built with AddressSanitizer and UndefinedBehaviorSanitizer
-    (g++, host code only) and run: random, index-reuse and adversarial trees,
-    and the reference's recorded Tree trace.  Any report fails the run."""
+    """The tile planner (tree_plan.hpp), the tree bookkeeping (TreeLayout in
+    host/Tree.cpp) and the serving loop's host code (host/Serving.cpp, with the
+    concurrent callers of serve_stress.cpp against the host-only runtime of
+    tests/cpp/host_runtime) built with AddressSanitizer and
+    UndefinedBehaviorSanitizer (g++, host code only) and run: random,
+    index-reuse and adversarial trees, the reference's recorded Tree trace, and
+    serving batches whose deltas are gathered on one and on several threads.
+    Any report fails the run."""
     import os
     subprocess.run(["make", "-s", "-C", str(ROOT / "tests" / "cpp"), "sanitize"], check=True, timeout=600)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1", UBSAN_OPTIONS="halt_on_error=1")
-    for args in (["asan_tree_plan_test", "--small"], ["asan_tree_layout_test", str(GOLDEN / "ref_tree_trace.bin")]):
+    for args in (["asan_tree_plan_test", "--small"], ["asan_tree_layout_test", str(GOLDEN / "ref_tree_trace.bin")],
+                 ["asan_serve_host"]):
         res = subprocess.run([str(BIN / args[0]), *args[1:]], capture_output=True, text=True, timeout=600,
                              cwd=str(ROOT), env=env)
         assert res.returncode == 0, res.stdout[-2000:] + res.stderr[-4000:]
