@@ -63,6 +63,8 @@ def parse():
                     help="launch only a 512 MiB device copy, the batched kernel, the flat kernel, the tree "
                          "propagation and the bounding box a few times each (for rocprofv3 --pmc passes); prints no JSON")
     ap.add_argument("--serve-only", action="store_true", help="only the serving measurement (prints its JSON)")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="only the issue rate of independent FP32 adds and multiplies (prints its JSON)")
     ap.add_argument("--serve-streams", type=int, default=8, help="streams of the serving measurement")
     ap.add_argument("--serve-depth", type=int, default=2, help="batches in flight per stream in the serving measurement")
     ap.add_argument("--serve-direct", action="store_true",
@@ -533,6 +535,19 @@ def pmc_probe():
     log("pmc probe: 3 copies of 512 MiB, 5 batched launches, 5 flat launches, 5 tree propagations")
 
 
+def valu_rates(lib, iters=20000):
+    """T lane-op/s of independent, unfused, unpacked FP32 adds and multiplies on
+    the device (dpc_probe_valu_rate): the issue rate the batched cull's
+    operations are measured against, beside the 256 CU x 64 lanes x 2.4 GHz model."""
+    from pipeline_amd import _lib
+    out = {}
+    for name, op in (("add", 0), ("mul", 1)):
+        r = C.c_double()
+        _lib.check(lib.dpc_probe_valu_rate(op, iters, C.byref(r), None))
+        out[f"fp32_{name}_T_lane_ops_per_s"] = r.value / 1e12
+    return out
+
+
 def main():
     args = parse()
     if args.pmc_probe:
@@ -542,6 +557,10 @@ def main():
         from oracle import oracle as orc
         from pipeline_amd import _lib
         print(json.dumps(bench_serving(args, _lib.load(), orc)))
+        return
+    if args.valu_probe:
+        from pipeline_amd import _lib
+        print(json.dumps(valu_rates(_lib.load())))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (requested --gpus {args.gpus}); "
@@ -628,6 +647,9 @@ def main():
     value = tests / (wall_b / args.steps)
     per_view_value = tests / (wall_v / args.steps)
     achieved_tops = ops / (k_mean_b * 1e-3) / 1e12
+    probe = valu_rates(lib)
+    log("VALU probe: independent FP32 adds {:.2f}, multiplies {:.2f} T lane-op/s (model {:.1f})".format(
+        probe["fp32_add_T_lane_ops_per_s"], probe["fp32_mul_T_lane_ops_per_s"], ISSUE_CEILING_TOPS))
     flops = FLOPS_PER_OBJECT * referenced + FLOPS_PER_TEST * tests
     achieved_tflops = flops / (k_mean_b * 1e-3) / 1e12
     log(f"batched: {k_mean_b * 1e3:.1f} us/launch (event), {wall_b / args.steps * 1e6:.1f} us/step wall "
@@ -857,10 +879,11 @@ def main():
                               "over the event-timed kernel; peak = published FP32 vector peak",
                      "issue_ceiling": {"achieved": achieved_tops, "peak": ISSUE_CEILING_TOPS, "unit": "T VALU op/s",
                                        "frac": achieved_tops / ISSUE_CEILING_TOPS,
+                                       "measured_fp32_add_mul": probe,
                                        "basis": f"{OPS_PER_OBJECT} ops/object + {OPS_PER_TEST} ops/test (add, mul "
                                                 "and compare); 256 CU x 64 lanes x 2.4 GHz, no FMA, no packed FP32. "
-                                                "Independent FP32 adds or muls alone issue at 38.5-38.8 T lane-op/s "
-                                                "here; compares issue faster, so about 1.0 is the issue limit of this "
+                                                "Independent FP32 adds or muls alone issue at the measured rates "
+                                                "below; compares issue faster, so about 1.0 is the issue limit of this "
                                                 "instruction mix (profiles/cull_batch_kernel_valu.md)"},
                      "hbm": {"achieved": hbm_bytes / (k_mean_b * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                              "frac": hbm_bytes / (k_mean_b * 1e-3) / 1e9 / PEAK_HBM_GBS,

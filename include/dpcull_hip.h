@@ -64,6 +64,13 @@ const char* dpc_last_error(void);
  * injection off.  Lets tests drive every error path of the host code (the C++
  * mirrors turn it into std::runtime_error) and check the state afterwards. */
 int dpc_debug_inject_error(int64_t after_calls, int hip_error);
+/* Calibration probe (no reference counterpart): the device's issue rate of
+ * independent FP32 adds (op 0) or multiplies (op 1), unfused and unpacked as
+ * every operation of the culling kernels is.  Eight independent accumulators per
+ * lane for `iters` rounds, 32 waves per CU, five launches timed with HIP events
+ * on `stream`; *lane_ops_per_s = lane operations per second.  The batched
+ * cull's issue fraction is read against it. */
+int dpc_probe_valu_rate(int op, uint32_t iters, double* lane_ops_per_s, void* stream);
 int dpc_device_info(int device, char* name, size_t name_len, int* cu_count, int* clock_khz,
                     size_t* total_mem, int* l2_bytes);
 int dpc_set_device(int device);

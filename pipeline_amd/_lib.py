@@ -124,6 +124,7 @@ _SIGNATURES = {
     "dpc_result_diff_workspace": (SZ, [SZ]),
     "dpc_result_diff": (I, [P, P, SZ, P, P, I, P, P]),
     "dpc_result_diff_noalloc": (I, [P, P, SZ, P, P, I, P]),
+    "dpc_probe_valu_rate": (I, [I, C.c_uint32, C.POINTER(C.c_double), P]),
     "dpc_cull_diff_workspace": (SZ, [SZ]),
     "dpc_cull_diff": (I, [P, P, P, SZ, P, SZ, FP, P, P, P, P, I, P, P]),
     "dpc_bbox_workspace": (SZ, [SZ]),

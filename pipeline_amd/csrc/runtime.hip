@@ -50,6 +50,24 @@ int device_cu_count()
 }
 
 namespace {
+// The issue rate of unfused, unpacked FP32 adds (Op 0) or multiplies (Op 1):
+// eight independent accumulators per lane, so no operation waits for another.
+template <int Op>
+__global__ __launch_bounds__(256) void valu_probe_kernel(float* __restrict__ out, float b, uint32_t iters)
+{
+  float a0 = static_cast<float>(threadIdx.x), a1 = a0 + 1.f, a2 = a0 + 2.f, a3 = a0 + 3.f, a4 = a0 + 4.f,
+        a5 = a0 + 5.f, a6 = a0 + 6.f, a7 = a0 + 7.f;
+  for (uint32_t i = 0; i < iters; ++i) {
+    if (Op == 0) {
+      a0 = a0 + b; a1 = a1 + b; a2 = a2 + b; a3 = a3 + b; a4 = a4 + b; a5 = a5 + b; a6 = a6 + b; a7 = a7 + b;
+    } else {
+      a0 = a0 * b; a1 = a1 * b; a2 = a2 * b; a3 = a3 * b; a4 = a4 * b; a5 = a5 * b; a6 = a6 * b; a7 = a7 * b;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  out[static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
 __global__ void fill_u32_kernel(uint32_t* __restrict__ dst, uint32_t value, size_t n)
 {
   const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
@@ -154,6 +172,44 @@ int dpc_memcpy_d2d(void* d_dst, const void* d_src, size_t bytes, void* stream)
   if (bytes == 0) return DPC_OK;
   DPC_HIP_TRY(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)), "hipMemcpyAsync D2D");
   return DPC_OK;
+}
+
+int dpc_probe_valu_rate(int op, uint32_t iters, double* lane_ops_per_s, void* stream)
+{
+  if (!lane_ops_per_s || (op != 0 && op != 1) || iters == 0)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "dpc_probe_valu_rate: op 0 or 1, iters > 0 and an output");
+  *lane_ops_per_s = 0.0;
+  const unsigned blocks = static_cast<unsigned>(device_cu_count()) * 8;  // 32 waves per CU
+  hipStream_t s = as_stream(stream);
+  float* d_out = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto launch = [&]() {
+    if (op == 0)
+      hipLaunchKernelGGL(valu_probe_kernel<0>, dim3(blocks), dim3(256), 0, s, d_out, 1.0000001f, iters);
+    else
+      hipLaunchKernelGGL(valu_probe_kernel<1>, dim3(blocks), dim3(256), 0, s, d_out, 1.0000001f, iters);
+    return check_launch("valu_probe_kernel");
+  };
+  int rc = DPC_OK;
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&d_out), sizeof(float) * 256 * blocks);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) rc = set_error(DPC_ERR_HIP, "dpc_probe_valu_rate: %s", hipGetErrorString(e));
+  if (rc == DPC_OK) rc = launch();  // (first launch untimed)
+  if (rc == DPC_OK && (e = hipEventRecord(e0, s)) != hipSuccess) rc = set_error(DPC_ERR_HIP, "hipEventRecord");
+  for (int r = 0; r < 5 && rc == DPC_OK; ++r) rc = launch();
+  if (rc == DPC_OK && (e = hipEventRecord(e1, s)) != hipSuccess) rc = set_error(DPC_ERR_HIP, "hipEventRecord");
+  float ms = 0.f;
+  if (rc == DPC_OK && (e = hipEventSynchronize(e1)) != hipSuccess) rc = set_error(DPC_ERR_HIP, "hipEventSynchronize");
+  if (rc == DPC_OK && (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess)
+    rc = set_error(DPC_ERR_HIP, "hipEventElapsedTime");
+  if (rc == DPC_OK && ms > 0.f)
+    *lane_ops_per_s = 5.0 * blocks * 256.0 * 8.0 * static_cast<double>(iters) / (ms * 1e-3);
+  (void)hipStreamSynchronize(s);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (d_out) (void)hipFree(d_out);
+  return rc;
 }
 
 int dpc_fill_u32(uint32_t* d_dst, uint32_t value, size_t n, void* stream)

@@ -233,3 +233,19 @@ def test_plan_rejects_bad_pairs():
     bad = _lib.BatchConfig(3, 256, 8, 0)
     pairs = (_lib.CullPair * 1)(_lib.CullPair(0, 0, 0))
     assert lib.dpc_batch_plan_create(groups, 1, pairs, 1, 1, C.byref(bad), C.byref(plan)) == 1
+
+
+def test_valu_probe_rates_are_plausible():
+    """dpc_probe_valu_rate (the calibration the batched cull's issue fraction is
+    read against): independent FP32 adds and multiplies issue at a positive rate
+    no higher than 64 lanes per CU per clock at the device's clock."""
+    import ctypes as C
+    from pipeline_amd import device as dev
+    lib = _lib.load()
+    info = dev.device_info(0)
+    ceiling = info["cu_count"] * 64 * info["clock_khz"] * 1e3
+    for op in (0, 1):
+        r = C.c_double()
+        _lib.check(lib.dpc_probe_valu_rate(op, 2000, C.byref(r), None))
+        assert 0.25 * ceiling < r.value <= 1.02 * ceiling, (op, r.value, ceiling)
+    assert lib.dpc_probe_valu_rate(2, 10, C.byref(C.c_double()), None) == 1  # DPC_ERR_INVALID_ARGUMENT
