@@ -360,6 +360,8 @@ typedef struct dpc_server_stats {
    * [4] mask copies and completion event */
   double issue_part_max_us[5];
   uint64_t delta_direct;     /* requests whose delta the device read straight from pinned host memory */
+  uint64_t delta_packed;     /* copied delta matrices scattered from their first three columns alone
+                                (their request's every matrix had last column 0 0 0 1 exactly) */
 } dpc_server_stats;
 
 int dpc_server_create(const dpc_server_config* cfg, dpc_server** server);
@@ -380,7 +382,10 @@ int dpc_server_remove_scene(dpc_server* server, uint32_t scene_id);
  * others are copied first: gathered by the host with the batch's other deltas
  * into the batch's pinned arena and transferred in one DMA copy (matrix arrays
  * above 32 MiB, or batches whose deltas exceed the arena, are transferred
- * from the caller's arrays instead).  A masks_out
+ * from the caller's arrays instead).  Gathered matrices travel as their first
+ * three columns; their last columns (m[3], m[7], m[11], m[15]) travel too only
+ * in a batch where some gathered matrix's last column differs, bit for bit,
+ * from 0 0 0 1; either way the scene receives every matrix bit for bit.  A masks_out
  * whose mask_capacity_words lie within one pinned host allocation is written by
  * the device directly (one copy-out launch per batch, no DMA transfer); any
  * other masks_out receives a copy. */
@@ -392,6 +397,11 @@ int dpc_server_submit(dpc_server* server, uint32_t scene_id, const float* vps, u
  * A request whose batch failed to launch (status != DPC_OK) applied no delta. */
 int dpc_server_wait(dpc_server* server, uint64_t request_id, double timeout_ms, dpc_request_info* info);
 int dpc_server_stats_get(dpc_server* server, dpc_server_stats* stats);
+/* Copies a scene's current matrices (every delta of its completed requests
+ * applied) to the host: n_matrices (the scene's count) x 16 floats, row-major,
+ * as dpc_server_add_scene takes them.  DPC_ERR_STATE while the scene has queued
+ * or in-flight requests.  The server's scheduler waits for the copy. */
+int dpc_server_read_matrices(dpc_server* server, uint32_t scene_id, float* matrices, uint64_t n_matrices);
 /* The server's clock (microseconds), to schedule arrivals against. */
 double dpc_server_now_us(dpc_server* server);
 

@@ -57,7 +57,7 @@ class ServerStats(C.Structure):
                 ("delta_matrices", C.c_uint64), ("scene_bytes", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
                 ("busy_us", C.c_double), ("issue_us", C.c_double),
                 ("issue_max_us", C.c_double), ("issue_part_max_us", C.c_double * 5),
-                ("delta_direct", C.c_uint64)]
+                ("delta_direct", C.c_uint64), ("delta_packed", C.c_uint64)]
 
 
 class CullConfig(C.Structure):
@@ -142,6 +142,7 @@ _SIGNATURES = {
     "dpc_server_wait": (I, [P, U64, C.c_double, C.POINTER(RequestInfo)]),
     "dpc_server_stats_get": (I, [P, C.POINTER(ServerStats)]),
     "dpc_server_now_us": (C.c_double, [P]),
+    "dpc_server_read_matrices": (I, [P, U32, P, U64]),
 }
 
 _lib: C.CDLL | None = None

@@ -42,6 +42,14 @@ constexpr uint32_t kMaxHostScatterLd = (1u << 26) - 1;
 int launch_scatter_host(float* d_planes, size_t ld, const uint32_t* idx, const float* src, size_t count,
                         void* stream);
 
+// Scatters count delta matrices that crossed PCIe packed (device memory, 16-byte
+// aligned): matrix k as three 16-byte rows in d_packed, (row i's x y z, row 3's
+// component i) for i = 0, 1, 2, and its last column (m[3], m[7], m[11], m[15])
+// as 16 bytes in d_tails, or, with d_tails null, 0 0 0 1 for every matrix.
+// Every value is moved as its bits.
+int launch_scatter_affine(float* d_planes, size_t ld, const uint32_t* d_idx, const void* d_packed,
+                          const void* d_tails, size_t count, void* stream);
+
 // One request's masks to its caller's buffer in pinned host memory: `words`
 // 64-bit words from byte offset src_offset of the batch arena to dst (the
 // buffer's device-visible address).

@@ -194,6 +194,26 @@ __global__ __launch_bounds__(256) void scatter_host_kernel(float4* __restrict__ 
   }
 }
 
+// The scatter of a packed delta (launch_scatter_affine): one thread per matrix,
+// three 16-byte loads, the last column from tails or 0 0 0 1 (the bits of
+// 0.0f, 0.0f, 0.0f, 1.0f), four 16-byte stores, one to each row plane.
+__global__ __launch_bounds__(256) void scatter_affine_kernel(uint4* __restrict__ planes, size_t ld,
+                                                             const uint32_t* __restrict__ idx,
+                                                             const uint4* __restrict__ packed,
+                                                             const uint4* __restrict__ tails, size_t count)
+{
+  const size_t k = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const size_t s = idx[k];
+  const uint4 a = packed[3 * k], b = packed[3 * k + 1], c = packed[3 * k + 2];
+  const uint4 w = tails ? tails[k] : make_uint4(0u, 0u, 0u, 0x3f800000u);
+  if (s >= ld) return;
+  planes[s] = make_uint4(a.x, a.y, a.z, w.x);
+  planes[ld + s] = make_uint4(b.x, b.y, b.z, w.y);
+  planes[2 * ld + s] = make_uint4(c.x, c.y, c.z, w.z);
+  planes[3 * ld + s] = make_uint4(a.w, b.w, c.w, w.w);
+}
+
 __global__ void gather_u32_kernel(const uint32_t* __restrict__ table, size_t table_len,
                                   const uint32_t* __restrict__ idx, size_t n, uint32_t* __restrict__ out)
 {
@@ -869,6 +889,20 @@ int launch_scatter_host(float* d_planes, size_t ld, const uint32_t* idx, const f
   hipLaunchKernelGGL(scatter_host_kernel, dim3(grid), dim3(256), 0, as_stream(stream),
                      reinterpret_cast<float4*>(d_planes), static_cast<uint32_t>(ld), idx, src, static_cast<uint32_t>(count));
   return check_launch("scatter_host_kernel");
+}
+
+int launch_scatter_affine(float* d_planes, size_t ld, const uint32_t* d_idx, const void* d_packed,
+                          const void* d_tails, size_t count, void* stream)
+{
+  if (count == 0) return DPC_OK;
+  if (!d_planes || !d_idx || !d_packed)
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "launch_scatter_affine: null pointer");
+  if (!aligned(d_planes, 16) || !aligned(d_packed, 16) || (d_tails && !aligned(d_tails, 16)))
+    return set_error(DPC_ERR_INVALID_ARGUMENT, "launch_scatter_affine: misaligned pointer");
+  hipLaunchKernelGGL(scatter_affine_kernel, dim3(grid_for(count, 256)), dim3(256), 0, as_stream(stream),
+                     reinterpret_cast<uint4*>(d_planes), ld, d_idx, static_cast<const uint4*>(d_packed),
+                     static_cast<const uint4*>(d_tails), count);
+  return check_launch("scatter_affine_kernel");
 }
 
 }  // namespace dpc

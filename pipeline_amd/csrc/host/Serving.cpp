@@ -27,6 +27,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <immintrin.h>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -56,16 +57,60 @@ constexpr uint64_t kStageMatsMax = kHostStageMax;
 
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// One host copy of a gather: bytes from src to dst.
+// One host copy of a gather: bytes from src to dst or, with tail set, bytes / 64
+// matrices packed by pack_affine.
 struct Piece {
   char* dst;
   const char* src;
   uint64_t bytes;
+  char* tail = nullptr;
+  std::atomic<bool>* nonaffine = nullptr;  // set when a packed matrix's last column is not 0 0 0 1
 };
 
-// Copies every piece.  From 2 MiB on, the bytes are split into equal runs for
-// up to four threads (this one and up to three started here and joined before
-// returning): one thread's memcpy would hold up the batch's transfer.
+// Packs matrices [j0, j1) of a delta (16 floats each, row-major) as three
+// 16-byte rows in packed, (row i's x y z, row 3's component i) for i = 0, 1, 2,
+// and the last column (m[3], m[7], m[11], m[15]) as 16 bytes in tail; packed
+// and tail are 16-byte aligned.  Every value is moved as its bits (shuffles
+// only), with streaming stores: the DMA copy reads the arena from memory, and
+// the stores need not read its lines first.  Returns whether some matrix's
+// last column differs, bit for bit, from 0 0 0 1 (0.0f, 0.0f, 0.0f, 1.0f).
+bool pack_affine(const char* src, char* packed, char* tail, uint64_t j0, uint64_t j1)
+{
+  const __m128i affine = _mm_set_epi32(0x3f800000, 0, 0, 0);
+  __m128i other = _mm_setzero_si128();
+  for (uint64_t j = j0; j < j1; ++j) {
+    const char* m = src + 64 * j;
+    const __m128 r0 = _mm_castsi128_ps(_mm_loadu_si128(reinterpret_cast<const __m128i*>(m)));
+    const __m128 r1 = _mm_castsi128_ps(_mm_loadu_si128(reinterpret_cast<const __m128i*>(m + 16)));
+    const __m128 r2 = _mm_castsi128_ps(_mm_loadu_si128(reinterpret_cast<const __m128i*>(m + 32)));
+    const __m128 r3 = _mm_castsi128_ps(_mm_loadu_si128(reinterpret_cast<const __m128i*>(m + 48)));
+    // row i's z and row 3's component i, then (x, y, z, row 3's component i)
+    const __m128 z0 = _mm_shuffle_ps(r0, r3, _MM_SHUFFLE(0, 0, 2, 2));
+    const __m128 z1 = _mm_shuffle_ps(r1, r3, _MM_SHUFFLE(1, 1, 2, 2));
+    const __m128 z2 = _mm_shuffle_ps(r2, r3, _MM_SHUFFLE(2, 2, 2, 2));
+    const __m128 p0 = _mm_shuffle_ps(r0, z0, _MM_SHUFFLE(2, 0, 1, 0));
+    const __m128 p1 = _mm_shuffle_ps(r1, z1, _MM_SHUFFLE(2, 0, 1, 0));
+    const __m128 p2 = _mm_shuffle_ps(r2, z2, _MM_SHUFFLE(2, 0, 1, 0));
+    // the w of each row: (r0.w, r1.w, r2.w, r3.w)
+    const __m128 u = _mm_unpackhi_ps(r0, r1);  // r0.z r1.z r0.w r1.w
+    const __m128 v = _mm_unpackhi_ps(r2, r3);  // r2.z r3.z r2.w r3.w
+    const __m128i w = _mm_castps_si128(_mm_movehl_ps(v, u));
+    __m128i* dp = reinterpret_cast<__m128i*>(packed + 48 * j);
+    _mm_stream_si128(dp, _mm_castps_si128(p0));
+    _mm_stream_si128(dp + 1, _mm_castps_si128(p1));
+    _mm_stream_si128(dp + 2, _mm_castps_si128(p2));
+    _mm_stream_si128(reinterpret_cast<__m128i*>(tail + 16 * j), w);
+    other = _mm_or_si128(other, _mm_xor_si128(w, affine));
+  }
+  _mm_sfence();  // the streaming stores are visible before this thread reports done
+  return _mm_movemask_epi8(_mm_cmpeq_epi8(other, _mm_setzero_si128())) != 0xFFFF;
+}
+
+// Copies (or packs) every piece.  From 2 MiB on, the source bytes are split into
+// equal runs for up to four threads (this one and up to three started here and
+// joined before returning): one thread's memcpy would hold up the batch's
+// transfer.  A packed piece is split at whole matrices: a run packs the
+// matrices that start within it.
 void gather(const std::vector<Piece>& pieces)
 {
   uint64_t total = 0;
@@ -78,7 +123,11 @@ void gather(const std::vector<Piece>& pieces)
     uint64_t at = 0;
     for (const Piece& p : pieces) {
       const uint64_t a = std::max(lo, at), b = std::min(hi, at + p.bytes);
-      if (a < b) std::memcpy(p.dst + (a - at), p.src + (a - at), b - a);
+      if (a < b && !p.tail) {
+        std::memcpy(p.dst + (a - at), p.src + (a - at), b - a);
+      } else if (a < b && pack_affine(p.src, p.dst, p.tail, (a - at + 63) / 64, (b - at + 63) / 64)) {
+        p.nonaffine->store(true, std::memory_order_relaxed);
+      }
       at += p.bytes;
       if (at >= hi) break;
     }
@@ -125,6 +174,7 @@ struct Request {
   uint64_t* masks_dev = nullptr;  // masks_out lies in pinned host memory: the device writes it, at this address
   uint64_t words = 0;       // mask words per frustum
   uint64_t dev_bytes = 0;   // arena bytes this request needs
+  bool delta_packed = false;  // its delta crossed PCIe as 48 bytes per matrix
   double t_submit = 0, t_admit = 0, t_done = 0;
   uint32_t batch = 0, batch_requests = 0;
   int status = DPC_OK;
@@ -304,27 +354,47 @@ void dpc_server::issue_batch(StreamSlot& slot)
   const uint64_t desc_end = off;
   // The copied deltas: every index array and the matrices of deltas up to
   // kStageMatsMax bytes first (the part staged on the host, below), then the
-  // larger matrix arrays, each copied on its own.
-  std::vector<uint64_t> idx_off(pick.size()), mat_off(pick.size());
-  for (size_t k = 0; k < pick.size(); ++k) {
-    const Request& r = *pick[k];
-    const uint64_t n = r.didx_dev ? 0 : r.n_delta;
-    idx_off[k] = off;
-    off = align_up(off + 4ull * n, 256);
-    if (64ull * n <= kStageMatsMax) {
-      mat_off[k] = off;
-      off = align_up(off + 64ull * n, 256);
+  // larger matrix arrays, each copied on its own.  When the staged part fits
+  // the host arena its matrices are packed: their first three columns (48
+  // bytes) up to packed_end, then their last columns (16 bytes), which cross
+  // PCIe only in a batch where some matrix's last column is not 0 0 0 1.
+  std::vector<uint64_t> idx_off(pick.size()), mat_off(pick.size()), tail_off(pick.size());
+  uint64_t packed_end = 0, stage_end = 0;
+  auto lay_out = [&](bool pack) {
+    uint64_t o = desc_end;
+    for (size_t k = 0; k < pick.size(); ++k) {
+      const Request& r = *pick[k];
+      const uint64_t n = r.didx_dev ? 0 : r.n_delta;
+      idx_off[k] = o;
+      o = align_up(o + 4ull * n, 256);
+      if (64ull * n <= kStageMatsMax) {
+        mat_off[k] = o;
+        o = align_up(o + (pack ? 48ull : 64ull) * n, 256);
+      }
     }
-  }
-  const uint64_t stage_end = off;
-  for (size_t k = 0; k < pick.size(); ++k) {
-    const Request& r = *pick[k];
-    const uint64_t n = r.didx_dev ? 0 : r.n_delta;
-    if (64ull * n > kStageMatsMax) {
-      mat_off[k] = off;
-      off = align_up(off + 64ull * n, 256);
+    packed_end = o;
+    for (size_t k = 0; k < pick.size() && pack; ++k) {
+      const Request& r = *pick[k];
+      const uint64_t n = r.didx_dev ? 0 : r.n_delta;
+      if (64ull * n <= kStageMatsMax) {
+        tail_off[k] = o;
+        o = align_up(o + 16ull * n, 256);
+      }
     }
-  }
+    stage_end = o;
+    for (size_t k = 0; k < pick.size(); ++k) {
+      const Request& r = *pick[k];
+      const uint64_t n = r.didx_dev ? 0 : r.n_delta;
+      if (64ull * n > kStageMatsMax) {
+        mat_off[k] = o;
+        o = align_up(o + 64ull * n, 256);
+      }
+    }
+    return o;
+  };
+  off = lay_out(true);
+  const bool staged = stage_end <= slot.h_cap;
+  if (!staged) off = lay_out(false);
   const uint64_t masks_off = off;
   auto* entries = reinterpret_cast<dpc::ServeEntry*>(slot.h_arena + entries_off);
   auto* pairs = reinterpret_cast<dpc::KPair*>(slot.h_arena + pairs_off);
@@ -366,11 +436,11 @@ void dpc_server::issue_batch(StreamSlot& slot)
   double t_part[6];
   double copy_us = 0.0;
   t_part[0] = now();
-  // The descriptors, the index arrays and the smaller matrix arrays staged
-  // together in the pinned host arena when they fit: one copy for them, then
-  // the larger matrix arrays on their own, then the scatters.  Otherwise each
-  // delta is copied on its own, before its scatter.
-  const bool staged = stage_end <= slot.h_cap;
+  // The descriptors, the index arrays and the smaller matrix arrays (packed)
+  // staged together in the pinned host arena when they fit: one copy for them,
+  // then the larger matrix arrays on their own, then the scatters.  Otherwise
+  // each delta is copied on its own, before its scatter.
+  std::vector<std::atomic<bool>> nonaffine(pick.size());  // (all false)
   if (staged) {
     std::vector<Piece> pieces;
     for (size_t k = 0; k < pick.size(); ++k) {
@@ -378,11 +448,14 @@ void dpc_server::issue_batch(StreamSlot& slot)
       if (!r.n_delta || r.didx_dev) continue;
       pieces.push_back(Piece{slot.h_arena + idx_off[k], reinterpret_cast<const char*>(r.didx), 4ull * r.n_delta});
       if (64ull * r.n_delta <= kStageMatsMax)
-        pieces.push_back(Piece{slot.h_arena + mat_off[k], reinterpret_cast<const char*>(r.dmats), 64ull * r.n_delta});
+        pieces.push_back(Piece{slot.h_arena + mat_off[k], reinterpret_cast<const char*>(r.dmats), 64ull * r.n_delta,
+                               slot.h_arena + tail_off[k], &nonaffine[k]});
     }
     gather(pieces);
+    bool tails = false;
+    for (auto& f : nonaffine) tails = tails || f.load(std::memory_order_relaxed);
     if (rc == DPC_OK) {
-      e = hipMemcpyAsync(slot.d_arena, slot.h_arena, stage_end, hipMemcpyHostToDevice, s);
+      e = hipMemcpyAsync(slot.d_arena, slot.h_arena, tails ? stage_end : packed_end, hipMemcpyHostToDevice, s);
       if (e != hipSuccess) rc = hip_fail(e, "batch upload");
     }
     copy_us = now() - t_part[0];
@@ -395,7 +468,16 @@ void dpc_server::issue_batch(StreamSlot& slot)
       rc = dpc::launch_scatter_host(snapshot[k].planes, snapshot[k].n_mat, r.didx_dev, r.dmats_dev, r.n_delta, s);
       continue;
     }
-    if (!staged || 64ull * r.n_delta > kStageMatsMax) {
+    if (staged && 64ull * r.n_delta <= kStageMatsMax) {
+      const bool tail = nonaffine[k].load(std::memory_order_relaxed);
+      rc = dpc::launch_scatter_affine(snapshot[k].planes, snapshot[k].n_mat,
+                                      reinterpret_cast<const uint32_t*>(slot.d_arena + idx_off[k]),
+                                      slot.d_arena + mat_off[k], tail ? slot.d_arena + tail_off[k] : nullptr,
+                                      r.n_delta, s);
+      if (!tail) pick[k]->delta_packed = true;
+      continue;
+    }
+    {
       const double tc = now();
       if (!staged) e = hipMemcpyAsync(slot.d_arena + idx_off[k], r.didx, 4ull * r.n_delta, hipMemcpyHostToDevice, s);
       if (e == hipSuccess)
@@ -480,6 +562,7 @@ void dpc_server::complete(StreamSlot& slot)
     if (r.status == DPC_OK) {
       stats.delta_matrices += r.n_delta;
       if (r.n_delta && r.didx_dev) stats.delta_direct++;
+      if (r.delta_packed) stats.delta_packed += r.n_delta;
     }
     r.t_done = t;
     r.done = true;
@@ -788,9 +871,10 @@ int dpc_server_submit(dpc_server* s, uint32_t scene_id, const float* vps, uint32
     return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_submit: the mask buffer is smaller than n_frusta x ceil(n/64) words");
   const uint64_t staged_delta = r->didx_dev ? 0 : n_delta;  // a pinned delta takes no arena space
   // (the last term: a batch's alignment to 256 bytes of its four descriptor
-  // regions and of this request's two delta regions)
+  // regions and of this request's three delta regions: indices, and matrices
+  // packed in two or whole in one)
   r->dev_bytes = align_up(4ull * staged_delta, 256) + align_up(64ull * staged_delta, 256) + sizeof(dpc::ServeEntry) +
-                 sizeof(dpc::CopyOut) + (sizeof(dpc::KPair) + 64) * n_frusta + 8 * r->words * n_frusta + 6 * 256;
+                 sizeof(dpc::CopyOut) + (sizeof(dpc::KPair) + 64) * n_frusta + 8 * r->words * n_frusta + 7 * 256;
   if (r->dev_bytes > s->slots[0].cap)
     return fail(DPC_ERR_OVER_BUDGET, "dpc_server_submit: the request alone exceeds the in-flight budget");
   r->id = s->next_id++;
@@ -833,6 +917,27 @@ int dpc_server_stats_get(dpc_server* s, dpc_server_stats* st)
   if (!s || !st) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_stats_get: null argument");
   std::lock_guard<std::mutex> g(s->mu);
   *st = s->stats;
+  return DPC_OK;
+}
+
+int dpc_server_read_matrices(dpc_server* s, uint32_t id, float* matrices, uint64_t n_matrices)
+{
+  if (!s || !matrices) return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_read_matrices: null argument");
+  // (the lock stays held over the copy: no request of the scene can be admitted meanwhile)
+  std::lock_guard<std::mutex> g(s->mu);
+  if (id >= s->scenes.size() || !s->scenes[id].live || !s->scenes[id].ready)
+    return fail(DPC_ERR_NOT_FOUND, "no such scene");
+  const Scene& sc = s->scenes[id];
+  if (sc.inflight || sc.pending)
+    return fail(DPC_ERR_STATE, "dpc_server_read_matrices: the scene has requests in flight");
+  if (n_matrices != sc.n_mat)
+    return fail(DPC_ERR_INVALID_ARGUMENT, "dpc_server_read_matrices: the scene holds another number of matrices");
+  // the four row planes (row r of matrix k at planes[4 * (r * n + k)]), then row-major per matrix
+  std::vector<float> rows(16 * sc.n_mat);
+  const hipError_t e = hipMemcpy(rows.data(), sc.planes, 64 * sc.n_mat, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "dpc_server_read_matrices");
+  for (uint64_t k = 0; k < sc.n_mat; ++k)
+    for (int r = 0; r < 4; ++r) std::memcpy(matrices + 16 * k + 4 * r, rows.data() + 4 * (r * sc.n_mat + k), 16);
   return DPC_OK;
 }
 

@@ -74,6 +74,7 @@ class Server:
         self.handle = C.c_void_p()
         check(self.lib.dpc_server_create(C.byref(cfg), C.byref(self.handle)))
         self.scene_objects: dict[int, int] = {}
+        self.scene_matrices: dict[int, int] = {}
         self._borrowed: dict[int, tuple] = {}  # request id -> arrays the server reads until the request is done
         self._scene_of: dict[int, int] = {}
 
@@ -89,11 +90,20 @@ class Server:
         check(self.lib.dpc_server_add_scene(self.handle, lu.ctypes.data, lu.shape[0], m.ctypes.data, m.shape[0],
                                             None if x is None else x.ctypes.data, C.byref(sid)))
         self.scene_objects[sid.value] = int(lu.shape[0])
+        self.scene_matrices[sid.value] = int(m.shape[0])
         return sid.value
 
     def remove_scene(self, scene: int) -> None:
         check(self.lib.dpc_server_remove_scene(self.handle, scene))
         del self.scene_objects[scene]
+        del self.scene_matrices[scene]
+
+    def read_matrices(self, scene: int) -> np.ndarray:
+        """The scene's current matrices (k x 16 float32, row-major), every delta of
+        its completed requests applied; the scene must have none in flight."""
+        out = np.empty((self.scene_matrices[scene], 16), np.float32)
+        check(self.lib.dpc_server_read_matrices(self.handle, scene, out.ctypes.data, out.shape[0]))
+        return out
 
     def mask_words(self, scene: int, n_frusta: int) -> int:
         return n_frusta * ((self.scene_objects[scene] + 63) // 64)

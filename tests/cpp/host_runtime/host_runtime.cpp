@@ -1,7 +1,9 @@
 // host_runtime.cpp -- the host-only stand-in declared in hip/hip_runtime_api.h
 // (this directory), plus the few library entry points the serving loop calls,
-// as no-ops on host memory.  Thread-safe: the serving loop calls it from its
-// scheduler, its issuers and its callers' threads at once.
+// on host memory: the matrix uploads and scatters move the matrices (so a
+// scene's matrices can be checked after a run), the culls compute nothing.
+// Thread-safe: the serving loop calls it from its scheduler, its issuers and
+// its callers' threads at once.
 #include "hip/hip_runtime_api.h"
 
 #include <atomic>
@@ -153,7 +155,30 @@ int launch_serve(const void*, uint32_t, uint64_t, const void*, const void*, cons
 {
   return DPC_OK;
 }
-int launch_scatter_host(float*, size_t, const uint32_t*, const float*, size_t, void*) { return DPC_OK; }
+// row r of matrix s lives at planes[4 * (r * ld + s)]
+int launch_scatter_host(float* planes, size_t ld, const uint32_t* idx, const float* src, size_t count, void*)
+{
+  for (size_t k = 0; k < count; ++k)
+    for (int r = 0; r < 4; ++r) std::memcpy(planes + 4 * (r * ld + idx[k]), src + 16 * k + 4 * r, 16);
+  return DPC_OK;
+}
+int launch_scatter_affine(float* planes, size_t ld, const uint32_t* idx, const void* packed, const void* tails,
+                          size_t count, void*)
+{
+  const uint32_t* p = static_cast<const uint32_t*>(packed);
+  const uint32_t* t = static_cast<const uint32_t*>(tails);
+  const uint32_t affine[4] = {0u, 0u, 0u, 0x3f800000u};
+  for (size_t k = 0; k < count; ++k) {
+    const uint32_t* w = t ? t + 4 * k : affine;
+    const uint32_t* a = p + 12 * k;
+    const uint32_t rows[4][4] = {{a[0], a[1], a[2], w[0]},
+                                 {a[4], a[5], a[6], w[1]},
+                                 {a[8], a[9], a[10], w[2]},
+                                 {a[3], a[7], a[11], w[3]}};
+    for (int r = 0; r < 4; ++r) std::memcpy(planes + 4 * (r * ld + idx[k]), rows[r], 16);
+  }
+  return DPC_OK;
+}
 int launch_copy_out(const void*, uint32_t, const char*, uint64_t, void*) { return DPC_OK; }
 }  // namespace dpc
 
@@ -162,6 +187,15 @@ const char* dpc_last_error(void) { return t_error; }
 int dpc_host_alloc(void** h_ptr, size_t bytes) { return hipHostMalloc(h_ptr, bytes, 0) == hipSuccess ? DPC_OK : 2; }
 int dpc_host_free(void* h_ptr) { return hipHostFree(h_ptr) == hipSuccess ? DPC_OK : 2; }
 int dpc_boxes_from_lower_upper(const float*, size_t, float*, float*, void*) { return DPC_OK; }
-int dpc_matrices_from_aos(const void*, size_t, size_t, float*, size_t, void*) { return DPC_OK; }
-int dpc_matrices_scatter(float*, size_t, const uint32_t*, const float*, size_t, void*) { return DPC_OK; }
+int dpc_matrices_from_aos(const void* aos, size_t stride_bytes, size_t count, float* planes, size_t ld, void*)
+{
+  for (size_t k = 0; k < count; ++k)
+    for (int r = 0; r < 4; ++r)
+      std::memcpy(planes + 4 * (r * ld + k), static_cast<const char*>(aos) + k * stride_bytes + 16 * r, 16);
+  return DPC_OK;
+}
+int dpc_matrices_scatter(float* planes, size_t ld, const uint32_t* idx, const float* src, size_t count, void* stream)
+{
+  return dpc::launch_scatter_host(planes, ld, idx, src, count, stream);
+}
 }

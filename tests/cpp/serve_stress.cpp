@@ -5,14 +5,19 @@
 // polls the statistics.  The scheduler and the per-stream issuers run beside
 // them.  Every request must succeed, and its masks must equal those of the
 // same requests run one at a time on a fresh server (a scene's requests apply
-// their deltas in submission order, so each scene's results are fixed).  Run
-// for both delta paths (copied and read directly from pinned memory).
+// their deltas in submission order, so each scene's results are fixed), and
+// every scene's matrices after either run must equal, bit for bit, its initial
+// matrices with every delta applied in order; some deltas hold matrices whose
+// last column is not 0 0 0 1 (a negative zero, a NaN, a projective term, a
+// homogeneous scale).  Run for both delta paths (copied and read directly from
+// pinned memory).
 //
-// Built with -DDPCULL_HOST_RUNTIME and ThreadSanitizer against the host-only
-// runtime of tests/cpp/host_runtime (bin/tsan_serve_host, on the CPU), the same
-// threads run the server's scheduler and issuers with no device: no masks are
-// computed then, and the run checks the statuses and that ThreadSanitizer finds
-// no data race among the threads.
+// Built with -DDPCULL_HOST_RUNTIME and ThreadSanitizer or AddressSanitizer
+// against the host-only runtime of tests/cpp/host_runtime (bin/tsan_serve_host,
+// bin/asan_serve_host, on the CPU), the same threads run the server's scheduler
+// and issuers with no device: the runtime moves the matrices but computes no
+// masks, and the run checks the statuses, the matrices and that the sanitizer
+// finds no data race or bad access among the threads.
 #include <atomic>
 #include <cstdlib>
 #include <cstdio>
@@ -60,6 +65,18 @@ static void identity_into(float* m)
 {
   std::memset(m, 0, 64);
   m[0] = m[5] = m[10] = m[15] = 1.f;
+}
+
+// A last column other than 0 0 0 1, of one of four kinds.
+static void set_last_column(float* m, uint32_t kind)
+{
+  const uint32_t nan_bits = 0x7fc01234u;  // a quiet NaN with a payload
+  switch (kind % 4) {
+    case 0: m[3] = -0.0f; break;
+    case 1: std::memcpy(&m[15], &nan_bits, 4); break;
+    case 2: m[11] = 0.001f; break;
+    default: m[15] = 2.0f; break;
+  }
 }
 
 static std::vector<Scene> make_scenes(std::mt19937& rng, int count)
@@ -128,6 +145,7 @@ static std::vector<Req> make_requests(std::mt19937& rng, const std::vector<Scene
         float* m = &r.mats[16ull * d];
         identity_into(m);
         for (int c = 0; c < 3; ++c) m[12 + c] = 20.f * u(rng);
+        if (k % 7 == 2 && d % 5 == 0) set_last_column(m, d / 5);
       }
     }
     r.masks.assign(r.n_frusta * ((sc.n + 63) / 64), 0);
@@ -189,11 +207,36 @@ static std::vector<uint32_t> add_scenes(dpc_server* s, const std::vector<Scene>&
   return ids;
 }
 
+// Each scene's matrices with every request's delta applied in request order.
+static std::vector<std::vector<float>> replay(const std::vector<Scene>& scenes, const std::vector<Req>& reqs)
+{
+  std::vector<std::vector<float>> out;
+  for (const auto& sc : scenes) out.push_back(sc.matrices);
+  for (const auto& r : reqs)
+    for (uint32_t d = 0; d < r.n_delta; ++d) std::memcpy(&out[r.scene][16ull * r.idx[d]], r.mats + 16ull * d, 64);
+  return out;
+}
+
+// The number of scenes whose matrices on the server differ, in any bit, from want.
+static size_t matrices_differ(dpc_server* s, const std::vector<uint32_t>& ids,
+                              const std::vector<std::vector<float>>& want)
+{
+  size_t differ = 0;
+  for (size_t i = 0; i < ids.size(); ++i) {
+    std::vector<float> got(want[i].size());
+    const int rc = dpc_server_read_matrices(s, ids[i], got.data(), got.size() / 16);
+    EXPECT(rc == DPC_OK, "read_matrices: %s", dpc_last_error());
+    differ += std::memcmp(got.data(), want[i].data(), 4 * got.size()) != 0;
+  }
+  return differ;
+}
+
 static void run(uint32_t direct)
 {
   std::mt19937 rng(17 + direct);
   const std::vector<Scene> scenes = make_scenes(rng, 6);
   std::vector<Req> reqs = make_requests(rng, scenes, 180);
+  const std::vector<std::vector<float>> want_matrices = replay(scenes, reqs);
 
   // one at a time on a server of one stream: the expected masks
   std::vector<std::vector<uint64_t>> want(reqs.size());
@@ -212,6 +255,8 @@ static void run(uint32_t direct)
       r.collect();
       want[k] = r.masks;
     }
+    const size_t bad = matrices_differ(s, ids, want_matrices);
+    EXPECT(bad == 0, "one at a time: %zu scenes' matrices differ from the replayed deltas", bad);
     dpc_server_destroy(s);
   }
 
@@ -264,6 +309,9 @@ static void run(uint32_t direct)
   poller.join();
   dpc_server_stats st{};
   dpc_server_stats_get(s, &st);
+  const size_t bad_matrices = matrices_differ(s, ids, want_matrices);
+  EXPECT(bad_matrices == 0, "%zu scenes' matrices differ from the replayed deltas", bad_matrices);
+  EXPECT(st.delta_packed > 0, "no delta was scattered packed");
   dpc_server_destroy(s);
   size_t differ = 0;
 #ifndef DPCULL_HOST_RUNTIME  // (the host-only runtime computes no masks)
@@ -271,9 +319,11 @@ static void run(uint32_t direct)
 #endif
   EXPECT(differ == 0, "%zu of %zu requests differ from the one-at-a-time run", differ, reqs.size());
   EXPECT(st.requests_done == reqs.size(), "requests_done %llu", static_cast<unsigned long long>(st.requests_done));
-  std::printf("  %s deltas: %zu requests on 6 scenes from 3 threads, %llu batches, %llu read directly, %zu differ\n",
+  std::printf("  %s deltas: %zu requests on 6 scenes from 3 threads, %llu batches, %llu read directly, "
+              "%llu matrices packed, %zu differ\n",
               direct ? "direct" : "copied", reqs.size(), static_cast<unsigned long long>(st.batches),
-              static_cast<unsigned long long>(st.delta_direct), differ);
+              static_cast<unsigned long long>(st.delta_direct), static_cast<unsigned long long>(st.delta_packed),
+              differ);
   free_requests(reqs);
 }
 

@@ -4,8 +4,9 @@ scenes and overlapped on several streams.
 
 Parity: every request's masks equal the oracle's cull of its scene with the
 deltas of that scene's earlier requests applied (host replay in submission
-order), bit for bit.  The same request stream run one request at a time per
-scene (alone) gives the same masks as the mixed, batched run."""
+order), bit for bit, and so do the scenes' matrices read back after a run.  The
+same request stream run one request at a time per scene (alone) gives the same
+masks as the mixed, batched run."""
 import numpy as np
 import pytest
 
@@ -35,9 +36,10 @@ class Replay:
         return np.concatenate([self.orc.cull_obbs(obbs, vp) for vp in req.vps])
 
 
-def _run_mix(scenes, reqs, streams=3, max_batch=3, direct_delta_reads=False, pinned_masks=False):
+def _run_mix(scenes, reqs, streams=3, max_batch=3, direct_delta_reads=False, pinned_masks=False, matrices_out=None):
     """Every request's (info, masks); with pinned_masks, every other request's
-    masks go to a buffer in pinned memory, which the device writes itself."""
+    masks go to a buffer in pinned memory, which the device writes itself.  A
+    list passed as matrices_out receives every scene's matrices after the run."""
     out, held = [], []
     try:
         with Server(max_scenes=16, max_batch_requests=max_batch, streams=streams, hbm_budget_bytes=4 << 30,
@@ -56,6 +58,8 @@ def _run_mix(scenes, reqs, streams=3, max_batch=3, direct_delta_reads=False, pin
             for rid, m in pending:
                 out.append((srv.wait(rid, 60_000.0), np.array(m, copy=True)))
             stats = srv.stats()
+            if matrices_out is not None:
+                matrices_out.extend(srv.read_matrices(i) for i in ids)
     finally:
         for pa in held:
             pa.free()
@@ -139,6 +143,7 @@ def test_serve_large_deltas(orc):
             pending.append((srv.submit(ids[r.scene], r.vps, m, r.delta_idx, r.delta_mats), m))
         for rid, m in pending:
             out.append((srv.wait(rid, 120_000.0), m))
+        got_mats = [srv.read_matrices(i) for i in ids]
     assert all(info.status == 0 for info, _ in out)
     print("batches:", [(info.batch, info.batch_requests) for info, _ in out])
     rep = Replay(orc, scenes)
@@ -149,6 +154,8 @@ def test_serve_large_deltas(orc):
         bad = sum(mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
                   for f in range(r.vps.shape[0]))
         assert bad == 0, f"request {i} ({r.delta_idx.size} matrices): {bad} bits differ from the oracle"
+    for s, got in enumerate(got_mats):
+        assert np.array_equal(got.view(np.uint32), rep.mats[s].view(np.uint32)), f"scene {s}: matrices differ"
 
 
 @pytest.mark.parametrize("direct_reads", [True, False], ids=["direct_reads", "copied"])
@@ -192,6 +199,76 @@ def test_serve_pinned_deltas_read_directly(orc, direct_reads):
     for p in pinned:
         p.free()
     assert bad == 0, f"{bad} visibility bits differ from the oracle"
+
+
+def _with_last_columns(mats, every, start):
+    """A copy of mats (k, 4, 4) whose every `every`-th matrix from `start` has a
+    last column other than 0 0 0 1: a negative zero, a NaN with a payload, a
+    projective term or a homogeneous scale, in turn."""
+    m = np.array(mats, np.float32, copy=True).reshape(-1, 16)
+    u = m.view(np.uint32)
+    for j, i in enumerate(range(start, m.shape[0], every)):
+        if j % 4 == 0:
+            u[i, 3] = 0x80000000
+        elif j % 4 == 1:
+            u[i, 15] = 0x7FC01234
+        elif j % 4 == 2:
+            m[i, 11] = np.float32(0.001)
+        else:
+            m[i, 15] = np.float32(2.0)
+    return m.reshape(-1, 4, 4)
+
+
+@pytest.mark.parametrize("direct_reads", [False, True], ids=["copied", "direct_reads"])
+def test_serve_packed_deltas_bit_exact(orc, direct_reads):
+    """Copied deltas cross PCIe as the first three columns of their matrices, and
+    their last columns only in a batch where some matrix's last column is not
+    exactly 0 0 0 1 (then only the requests holding such matrices read them).
+    Requests with affine deltas mixed in batches with requests whose deltas hold
+    negative zeros, NaNs with payloads, projective terms and homogeneous scales
+    in that column (and, with direct reads, deltas in pinned memory that are
+    read directly): every scene's matrices read back afterwards equal the
+    replayed deltas bit for bit, every request's masks equal the oracle's, and
+    exactly the copied all-affine deltas were scattered from three columns."""
+    scenes = S.serving_scenes(seed=31, n_scenes=5, size_range=(3_000, 60_000))
+    reqs = S.serving_requests(scenes, seed=32, n_requests=40, rate_per_s=0.0, frusta=(1, 4))
+    mixed, pinned, want_packed, want_direct = [], [], 0, 0
+    for i, r in enumerate(reqs):
+        idx, mats = r.delta_idx, r.delta_mats
+        if idx.size and i % 4 == 1:
+            mats = _with_last_columns(mats, 7, i % 7)
+        if idx.size and direct_reads and i % 3 == 0:
+            pi, pm = PinnedArray(idx.shape, np.uint32), PinnedArray(mats.shape, np.float32)
+            pi.array[:] = idx
+            pm.array[:] = mats
+            pinned += [pi, pm]
+            idx, mats = pi.array, pm.array
+            want_direct += 1
+        elif idx.size and i % 4 != 1:
+            want_packed += idx.size
+        mixed.append(S.ServingRequest(r.scene, r.vps, idx, mats, r.arrival_us))
+    got_mats = []
+    try:
+        mix, stats = _run_mix(scenes, mixed, streams=2, max_batch=4, direct_delta_reads=direct_reads,
+                              matrices_out=got_mats)
+        assert all(info.status == 0 for info, _ in mix)
+        assert max(info.batch_requests for info, _ in mix) > 1
+        assert stats["delta_direct"] == want_direct
+        assert stats["delta_packed"] == want_packed
+        rep = Replay(orc, scenes)  # (reads the pinned deltas: freed below)
+        bad = 0
+        for r, (info, m) in zip(mixed, mix):
+            want = rep.apply_and_cull(r)
+            n = scenes[r.scene].n
+            words = (n + 63) // 64
+            for f in range(r.vps.shape[0]):
+                bad += mismatches(m[f * words:(f + 1) * words], want[f * words:(f + 1) * words], n)
+    finally:
+        for p in pinned:
+            p.free()
+    assert bad == 0, f"{bad} visibility bits differ from the oracle"
+    for s, got in enumerate(got_mats):
+        assert np.array_equal(got.view(np.uint32), rep.mats[s].view(np.uint32)), f"scene {s}: matrices differ"
 
 
 def test_serve_budget_admission(orc):
@@ -275,6 +352,11 @@ def test_serve_errors_and_scene_slots(orc):
             k = 250_000
             srv.submit(a, vp, m, (np.arange(k) % sc.n).astype(np.uint32), np.zeros((k, 16), np.float32))
         assert e.value.code == _lib.DPC_ERR_OVER_BUDGET
+        # a scene's matrices read back as uploaded; an unknown scene, a wrong count
+        assert np.array_equal(srv.read_matrices(a).view(np.uint32), sc.matrices.reshape(-1, 16).view(np.uint32))
+        buf = np.empty((sc.n, 16), np.float32)
+        assert srv.lib.dpc_server_read_matrices(srv.handle, 7, buf.ctypes.data, sc.n) == _lib.DPC_ERR_NOT_FOUND
+        assert srv.lib.dpc_server_read_matrices(srv.handle, a, buf.ctypes.data, 3) == _lib.DPC_ERR_INVALID_ARGUMENT
         # remove a scene, add another into its slot, and cull it
         srv.remove_scene(b)
         with pytest.raises(_lib.DpcError) as e:
