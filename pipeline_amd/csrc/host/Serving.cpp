@@ -10,7 +10,9 @@
 // the stream that runs them and joins only batches of that stream: stream order
 // then keeps each scene's requests in submission order (each sees the deltas
 // of the earlier ones) without waiting for them to finish.  A batch copies the
-// deltas to the device and scatters them into the scenes' matrices, tests every
+// deltas to the device (gathered on the host into one copy, each matrix as its
+// first three columns when the batch's last columns are all 0 0 0 1) and
+// scatters them into the scenes' matrices, tests every
 // scene of the batch against its request's frusta in one launch
 // (dpc::launch_serve), then copies each request's masks to its caller's
 // buffer.  Batches on different streams overlap on the device.

@@ -237,6 +237,11 @@ def test_serve_packed_deltas_bit_exact(orc, direct_reads):
         idx, mats = r.delta_idx, r.delta_mats
         if idx.size and i % 4 == 1:
             mats = _with_last_columns(mats, 7, i % 7)
+        if idx.size and i % 5 == 2:  # matrices 4 bytes past a 16-byte boundary
+            buf = np.empty(mats.size + 1, np.float32)
+            mats = buf[1:].reshape(mats.shape)
+            mats[:] = r.delta_mats if i % 4 != 1 else _with_last_columns(r.delta_mats, 7, i % 7)
+            assert mats.ctypes.data % 16 == 4
         if idx.size and direct_reads and i % 3 == 0:
             pi, pm = PinnedArray(idx.shape, np.uint32), PinnedArray(mats.shape, np.float32)
             pi.array[:] = idx
