@@ -441,7 +441,9 @@ def bench_serving(args, lib, orc, rates=(2000, 5000, 10000, 20000)):
         p.free()
     return dict(scenes=16, objects=sum(sizes), scene_sizes=sizes, runs=runs, oracle_check=check, pinned_copy=pcie,
                 server={"streams": streams, "batches_per_stream": depth, "max_batch_requests": 16, "hbm_budget_bytes": 32 << 30,
-                        "deltas": "read directly from pinned memory" if direct else "copied (DMA)",
+                        "deltas": "read directly from pinned memory" if direct else
+                                  "gathered on the host and copied (DMA), affine matrices as 48 bytes",
+                        "delta_matrices": st["delta_matrices"], "delta_matrices_packed": st["delta_packed"],
                         "scene_bytes": st["scene_bytes"], "peak_inflight_bytes": st["peak_inflight_bytes"]})
 
 
