@@ -29,6 +29,7 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <immintrin.h>
 #include <memory>
 #include <mutex>
@@ -108,19 +109,109 @@ bool pack_affine(const char* src, char* packed, char* tail, uint64_t j0, uint64_
   return _mm_movemask_epi8(_mm_cmpeq_epi8(other, _mm_setzero_si128())) != 0xFFFF;
 }
 
-// Copies (or packs) every piece.  From 2 MiB on, the source bytes are split into
-// equal runs for up to four threads (this one and up to three started here and
-// joined before returning): one thread's memcpy would hold up the batch's
-// transfer.  A packed piece is split at whole matrices: a run packs the
-// matrices that start within it.
-void gather(const std::vector<Piece>& pieces)
+// Helper threads shared by the issuers' gathers.  A gather offers every part
+// but its first to the helpers, runs the first itself, then takes back and
+// runs each part that no helper has started yet, and waits for the parts that
+// helpers are running: it never waits for a helper to become free.
+class GatherPool {
+ public:
+  // Starts up to `helpers` threads (a thread that cannot be started is left out).
+  void start(unsigned helpers)
+  {
+    try {
+      for (unsigned k = 0; k < helpers; ++k) threads_.emplace_back([this] { loop(); });
+    } catch (...) {
+    }
+  }
+  // Joins the helpers; no gather may be running.
+  void stop()
+  {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto& t : threads_) t.join();
+    threads_.clear();
+  }
+  unsigned helpers() const { return static_cast<unsigned>(threads_.size()); }
+  // Runs fn(0), ..., fn(parts - 1), on this thread and the helpers.
+  void run(unsigned parts, const std::function<void(unsigned)>& fn)
+  {
+    if (parts <= 1 || threads_.empty()) {
+      for (unsigned t = 0; t < parts; ++t) fn(t);
+      return;
+    }
+    Job job{&fn, parts - 1};
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      for (unsigned t = 1; t < parts; ++t) tasks_.push_back(Task{&job, t});
+    }
+    cv_work_.notify_all();
+    fn(0);
+    for (;;) {
+      unsigned part = 0;
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        const auto it = std::find_if(tasks_.begin(), tasks_.end(), [&](const Task& k) { return k.job == &job; });
+        if (it == tasks_.end()) break;
+        part = it->part;
+        tasks_.erase(it);
+      }
+      fn(part);
+      std::lock_guard<std::mutex> g(mu_);
+      --job.pending;
+    }
+    // (a helper touches the job for the last time when it counts its part done, under the lock)
+    std::unique_lock<std::mutex> lock(mu_);
+    cv_done_.wait(lock, [&] { return job.pending == 0; });
+  }
+
+ private:
+  struct Job {
+    const std::function<void(unsigned)>* fn;
+    unsigned pending;  // parts not yet done, the first one excepted
+  };
+  struct Task {
+    Job* job;
+    unsigned part;
+  };
+  void loop()
+  {
+    std::unique_lock<std::mutex> lock(mu_);
+    for (;;) {
+      cv_work_.wait(lock, [&] { return stop_ || !tasks_.empty(); });
+      if (tasks_.empty()) return;  // (stopping)
+      const Task k = tasks_.front();
+      tasks_.pop_front();
+      lock.unlock();
+      (*k.job->fn)(k.part);
+      lock.lock();
+      if (--k.job->pending == 0) cv_done_.notify_all();
+    }
+  }
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<Task> tasks_;
+  std::vector<std::thread> threads_;
+  bool stop_ = false;
+};
+
+constexpr unsigned kGatherHelpers = 4;
+
+// Copies (or packs) every piece.  From 512 KiB on, the source bytes are split
+// into equal runs of at least 256 KiB for this thread and the pool's helpers:
+// one thread's memcpy would hold up the batch's transfer.  A packed piece is
+// split at whole matrices: a run packs the matrices that start within it.
+void gather(GatherPool& pool, const std::vector<Piece>& pieces)
 {
   uint64_t total = 0;
   for (const Piece& p : pieces) total += p.bytes;
-  constexpr uint64_t kSplitMin = 2ull << 20, kPerThreadMin = 1ull << 20;
-  const unsigned parts = total < kSplitMin ? 1u : static_cast<unsigned>(std::min<uint64_t>(4, total / kPerThreadMin));
+  constexpr uint64_t kSplitMin = 512ull << 10, kPerPartMin = 256ull << 10;
+  const unsigned parts =
+      total < kSplitMin ? 1u : static_cast<unsigned>(std::min<uint64_t>(1 + pool.helpers(), total / kPerPartMin));
   // part t copies the bytes [t * total / parts, (t + 1) * total / parts) of the concatenated pieces
-  auto run = [&](unsigned t) {
+  pool.run(parts, [&](unsigned t) {
     const uint64_t lo = total * t / parts, hi = total * (t + 1) / parts;
     uint64_t at = 0;
     for (const Piece& p : pieces) {
@@ -133,18 +224,7 @@ void gather(const std::vector<Piece>& pieces)
       at += p.bytes;
       if (at >= hi) break;
     }
-  };
-  // (a helper that cannot be started leaves its part to this thread)
-  std::vector<std::thread> helpers;
-  unsigned started = 1;
-  try {
-    helpers.reserve(parts - 1);
-    for (; started < parts; ++started) helpers.emplace_back(run, started);
-  } catch (...) {
-  }
-  for (unsigned t = started; t < parts; ++t) run(t);
-  run(0);
-  for (auto& h : helpers) h.join();
+  });
 }
 
 struct Scene {
@@ -234,6 +314,7 @@ struct dpc_server {
   uint64_t arena_total = 0;
   bool stop = false;
   std::thread worker;
+  GatherPool pool;  // helpers of the issuers' gathers
   std::vector<std::unique_ptr<Issuer>> issuers;  // one per stream
   bool issuers_stop = false;
   dpc_server_stats stats{};
@@ -453,7 +534,7 @@ void dpc_server::issue_batch(StreamSlot& slot)
         pieces.push_back(Piece{slot.h_arena + mat_off[k], reinterpret_cast<const char*>(r.dmats), 64ull * r.n_delta,
                                slot.h_arena + tail_off[k], &nonaffine[k]});
     }
-    gather(pieces);
+    gather(pool, pieces);
     bool tails = false;
     for (auto& f : nonaffine) tails = tails || f.load(std::memory_order_relaxed);
     if (rc == DPC_OK) {
@@ -697,6 +778,7 @@ int dpc_server_create(const dpc_server_config* cfg, dpc_server** out)
     const int index = static_cast<int>(k);
     s->issuers[k]->thread = std::thread([s, index] { s->issue_loop(index); });
   }
+  s->pool.start(kGatherHelpers);
   s->worker = std::thread([s] { s->run(); });
   *out = s;
   return DPC_OK;
@@ -722,6 +804,7 @@ int dpc_server_destroy(dpc_server* s)
     is->cv.notify_all();
     if (is->thread.joinable()) is->thread.join();
   }
+  s->pool.stop();  // (no issuer is left to gather)
   for (auto& st : s->streams)
     if (st) (void)hipStreamSynchronize(st);
   for (auto& sl : s->slots) {
