@@ -123,7 +123,8 @@ class GatherPool {
     } catch (...) {
     }
   }
-  // Joins the helpers; no gather may be running.
+  ~GatherPool() { stop(); }
+  // Joins the helpers (again: nothing to do); no gather may be running.
   void stop()
   {
     {
