@@ -63,3 +63,7 @@ def test_argument_errors_are_reported_before_any_device_call(lib):
     # dpc_inverse_transpose: more matrices than the leading dimension holds
     assert lib.dpc_inverse_transpose(a, 5, None, 10, a, 10, None, None) == inval
     assert b"exceeds" in lib.dpc_last_error()
+    # dpc_server_read_matrices: no server, no output array
+    assert lib.dpc_server_read_matrices(None, 0, a, 1) == inval
+    assert lib.dpc_server_read_matrices(a, 0, None, 1) == inval
+    assert b"null" in lib.dpc_last_error()
